@@ -1,0 +1,245 @@
+/*
+ * C ABI of the MI355X aerial-image library (libopticalimaging_amd).
+ *
+ * The reference (gary21978/OpticalImaging) is pure Python over torch and has no
+ * FFI: its imaging entry points are Python functions taking parameter objects.
+ * Each entry point below replaces the numerical core of one of them; the
+ * Python package `opticalimaging_amd` keeps the reference signatures and calls
+ * these through ctypes with device pointers of torch tensors.
+ *
+ * Conventions
+ *   - Every function returns an int status: OI_OK (0) or a negative code.
+ *     OI_ERR_INVALID maps to Python ValueError, the others to RuntimeError.
+ *     oi_last_error() returns the message of the last failure on the calling
+ *     thread.
+ *   - All array arguments are caller-owned device pointers.  Complex arrays are
+ *     interleaved float32 (re, im) pairs ("c64").  Layouts are row-major with
+ *     the last index fastest.
+ *   - Plans are library-owned; they hold twiddle tables and scratch sized at
+ *     creation.  Calls that take a plan do not allocate device memory.
+ *   - Work is enqueued on the given HIP stream; no call synchronises the device
+ *     unless its comment says so.  A plan is used by one host thread at a time.
+ *   - No C++ exception crosses this boundary.
+ *
+ * Frequency conventions (shared by all entry points)
+ *   A grid of ny x nx samples with pitches Period_Y, Period_X has spectrum
+ *   index kappa = (kr, kc), kr in [-(ny/2), ny-1-ny/2] (integer division), and
+ *   normalised frequency (kr / p_y, kc / p_x) with p = Period * NA / lambda.
+ *   Rows go with Period_Y and with the source X coordinate
+ *   (CalculateAerialImage.py:31,55; CalculateTCCMatrix.py:12,23).
+ *   A "window" is the rectangle kr in [k0r, k0r+wr), kc in [k0c, k0c+wc).
+ *   The windowed spectrum of a field h is
+ *       spec(kappa) = (1/(ny nx)) sum_m h[m] exp(+2 pi i kappa.m / n)
+ *   i.e. Scatter.CalculateSpectrum's Hk (Scatter.py:20-23) without its p_x p_y
+ *   factor and its fftshift phase.
+ */
+#ifndef OPTICALIMAGING_AMD_H
+#define OPTICALIMAGING_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OI_OK 0
+#define OI_ERR_INVALID (-1)     /* bad argument or configuration -> ValueError */
+#define OI_ERR_RUNTIME (-2)     /* HIP runtime failure -> RuntimeError         */
+#define OI_ERR_UNSUPPORTED (-3) /* valid request this build cannot serve       */
+#define OI_ERR_NOMEM (-4)       /* device allocation failed                    */
+
+/* Field encodings accepted by the spectrum entry points. */
+#define OI_FIELD_REAL 0         /* float32 [batch][ny][nx], one channel (thin mask)  */
+#define OI_FIELD_C64_CHLAST 1   /* c64 [batch][ny][nx][cin] (ScatterField layout)    */
+
+/* TCC contraction precisions. */
+#define OI_TCC_SPLIT_BF16 0     /* 3-term split-bf16 on MFMA, fp32 accumulation      */
+#define OI_TCC_FP32 1           /* fp32 MFMA                                         */
+#define OI_TCC_FP64 2           /* fp64 MFMA                                         */
+
+/* Last error message of the calling thread ("" if none). */
+const char* oi_last_error(void);
+
+/* Library version, major*10000 + minor*100 + patch. */
+int oi_version(void);
+
+/* ------------------------------------------------------------------ plan --
+ * Geometry of one imaging problem.  Inputs to oi_plan_create: ny, nx, the
+ * spectrum window, nchan (channels carried per image), max_batch, max_groups
+ * (kernels are split into this many partial sums, fixed for the plan so a
+ * result never depends on the batch it is computed in; 0 = choose) and
+ * optionally lr, lc (coarse grid; 0 = choose).
+ * Outputs filled by the library: max_groups (when 0), lr, lc, rows (coarse rows per workgroup),
+ * direct (1 when the coarse grid equals the image grid), qr, qc (half-widths
+ * of the intensity band carried from the coarse to the image grid).
+ *
+ * The coarse grid must satisfy lr >= 2*band_r - 1 where band_r is the widest
+ * row extent of any single coherent kernel's support (and likewise for
+ * columns); oi_plan_create takes band_r = wr, band_c = wc unless band_r/band_c
+ * are set (> 0), which the Abbe path uses (one shifted pupil per kernel). */
+typedef struct oi_geometry {
+  int ny, nx;
+  int k0r, wr, k0c, wc;
+  int band_r, band_c;
+  int nchan;
+  int max_batch;
+  int max_groups;
+  int lr, lc;
+  int rows;
+  int direct;
+  int qr, qc;
+} oi_geometry;
+
+typedef struct oi_plan oi_plan;
+
+int oi_plan_create(oi_geometry* geo, oi_plan** out);
+int oi_plan_destroy(oi_plan* plan);
+/* Device bytes held by the plan (twiddles + scratch). */
+size_t oi_plan_bytes(const oi_plan* plan);
+
+/* Per-stage timing.  With slots > 0 every later imaging call on the plan
+ * records HIP events on its stream at the boundaries of its five launches
+ * (spectrum rows | spectrum columns | coherent sum | interpolation columns |
+ * interpolation rows), keeping the last `slots` calls; slots = 0 turns it off.
+ * oi_plan_read_timing waits for the recorded events and returns, per stage,
+ * the summed milliseconds (stage_ms[5]) over *ncalls recorded calls.        */
+int oi_plan_set_timing(oi_plan* plan, int slots);
+int oi_plan_read_timing(oi_plan* plan, double* stage_ms, int* ncalls);
+
+/* Windowed spectrum of a batch of fields.
+ * Replaces the spectrum step of Scatter.CalculateSpectrum (Scatter.py:10-23)
+ * as used inside CalculateAerialImage.py:27 and CalculateAerialImage_SOCS.py:4.
+ *   field : OI_FIELD_REAL float32 [batch][ny][nx]  or
+ *           OI_FIELD_C64_CHLAST c64 [batch][ny][nx][cin]
+ *   chans : for C64_CHLAST, the nchan channel indices to transform (< cin);
+ *           ignored for REAL (nchan must be 1)
+ *   spec  : out, c64 [batch][nchan][wr][wc]                                  */
+int oi_spectrum_window(oi_plan* plan, const void* field, int kind, int cin,
+                       const int* chans, int batch, void* spec, void* stream);
+
+/* Coherent-sum image: out[b][y][x] = sum_c sum_k |sum_kappa
+ *   kern[k](kappa) spec[b][c](kappa) exp(-2 pi i kappa.(y,x)/(ny,nx))|^2.
+ * With kern = SOCS kernels this is CalculateAerialImage_SOCS
+ * (CalculateAerialImage_SOCS.py:5-10); with kern = weighted shifted pupils it
+ * is the source sum of CalculateAbbeImage (CalculateAerialImage.py:46-92).
+ *   spec : c64 [batch][nchan][wr][wc] (from oi_spectrum_window)
+ *   kern : c64 [nkern][wr][wc]
+ *   out  : float32 [batch][ny][nx]                                            */
+int oi_coherent_image(oi_plan* plan, const void* spec, int batch,
+                      const void* kern, int nkern, float* out, void* stream);
+
+/* oi_spectrum_window followed by oi_coherent_image; spec lives in the plan.
+ * Replaces CalculateAerialImage_SOCS (CalculateAerialImage_SOCS.py:3-11) for a
+ * batch of masks.                                                            */
+int oi_socs_image(oi_plan* plan, const void* field, int kind, int cin,
+                  const int* chans, int batch, const void* kern, int nkern,
+                  float* out, void* stream);
+
+/* Full centred spectrum Hk = scale * fftshift(ifft2(ifftshift(h))) of one
+ * batch of fields, written channels-last c64 [batch][ny][nx][nchan].
+ * Replaces Scatter.CalculateSpectrum's Hk (Scatter.py:20-23) with
+ * scale = p_x p_y.  work: c64 scratch of batch*nchan*ny*nx elements.
+ * Creates and frees its own twiddle tables (not for hot loops).            */
+int oi_spectrum_full(const void* field, int kind, int cin, const int* chans,
+                     int nchan, int batch, int ny, int nx, float scale,
+                     void* work, void* hk, void* stream);
+
+/* ------------------------------------------------------ SOCS kernel sets --
+ * Support of dense SOCS kernels c64 [ny][nx][nk] (DecomposeTCC_SOCS.py:24-35
+ * layout).  rowflag[ny], colflag[nx] (int32, zeroed by the caller) receive 1
+ * where any kernel value is non-zero.                                        */
+int oi_kernel_support(const void* dense, int ny, int nx, int nk, int* rowflag,
+                      int* colflag, void* stream);
+
+/* Window of dense kernels: out c64 [nk][wr][wc], out[k][i][j] =
+ * dense[(k0r+i) mod ny][(k0c+j) mod nx][k], i.e. kernel value at kappa in the
+ * layout CalculateAerialImage_SOCS.py:6 multiplies with ifftshift(Hk).      */
+int oi_kernel_window(const void* dense, int ny, int nx, int nk, int k0r, int wr,
+                     int k0c, int wc, void* out, void* stream);
+
+/* Dense kernels from windowed ones (inverse of oi_kernel_window; the rest of
+ * the dense array is zeroed).  Writes c64 [ny][nx][nk].                     */
+int oi_kernel_dense(const void* win, int nk, int k0r, int wr, int k0c, int wc,
+                    int ny, int nx, void* dense, void* stream);
+
+/* ----------------------------------------------------------- pupil / TCC --
+ * Optics of one focus plane (Projection.py:5-10, Source.py:18).            */
+typedef struct oi_optics {
+  double wavelength;      /* nm                                            */
+  double na;              /* numerical aperture                            */
+  double magnification;   /* reduction ratio M                             */
+  double index_image;     /* n                                             */
+  double focus;           /* nm                                            */
+  double zernike[37];     /* Fringe Z1..Z37, waves                         */
+  double pitch_r, pitch_c;/* p_y, p_x: Period * NA / wavelength            */
+  int vector;             /* 0 scalar, 1 vector (3 components)             */
+} oi_optics;
+
+/* Shifted-pupil kernels on a window.
+ * G[e][s][i][j] = scale[s] * P(kr/p_r + sx[s], kc/p_c + sy[s]) * pol_e,
+ * kr = k0r+i, kc = k0c+j, zero outside rho <= 1 (rho^2 decided in float64).
+ * P = obliquity * exp(2 pi i W) * defocus (CalculateAerialImage.py:62-67,
+ * CalculateTCCMatrix.py:25-38); pol_e = px[s] M_xe + py[s] M_ye in vector mode
+ * (CalculateAerialImage.py:68-74, CalculateTCCMatrix.py:40-46), 1 in scalar.
+ *   sx, sy, scale : float64 [nsrc];  px, py : c128 [nsrc] (vector mode only)
+ *   G : c64 [ncomp][nsrc][wr][wc], ncomp = 3 (vector) or 1 (scalar).
+ * Evaluated in float64, rounded to c64.                                      */
+int oi_pupil_kernels(const oi_optics* opt, const double* sx, const double* sy,
+                     const double* scale, const double* px, const double* py,
+                     int nsrc, int k0r, int wr, int k0c, int wc, void* G,
+                     void* stream);
+
+/* Hermitian TCC contraction (GetTCCMatrix, CalculateTCCMatrix.py:58-76, summed
+ * over components as at :48-51 and scaled as at :55):
+ *   tcc[m][m'] = alpha * sum_e sum_s w[s] G[e][s][m] conj(G[e][s][m'])
+ *   G : c64 [ncomp][nsrc][M]; w : float32 [nsrc]; tcc : c64 [M][M].
+ * precision: OI_TCC_SPLIT_BF16 | OI_TCC_FP32 | OI_TCC_FP64.
+ * work: scratch of oi_tcc_work_bytes(M, nsrc, ncomp, precision) bytes.       */
+size_t oi_tcc_work_bytes(int M, int nsrc, int ncomp, int precision);
+int oi_tcc_build(const void* G, const float* w, int M, int nsrc, int ncomp,
+                 double alpha, int precision, void* work, void* tcc,
+                 void* stream);
+
+/* Gram matrix of B = [sqrt(w) G_e^T]_e in float64:
+ *   H[(e,s)][(e',s')] = sqrt(w_s w_s') sum_m conj(G[e][s][m]) G[e'][s'][m]
+ *   G : c64 [ncomp][nsrc][M]; H : c128 [ncomp*nsrc][ncomp*nsrc].
+ * Non-zero eigenpairs of H are those of the TCC (DecomposeTCC_SOCS.py:7).   */
+int oi_tcc_gram(const void* G, const double* w, int M, int nsrc, int ncomp,
+                void* H, void* stream);
+
+/* Kernels from Gram eigenvectors: out[k][m] = sum_(e,s) sqrt(w_s) G[e][s][m]
+ * Q[(e,s)][k], float64 accumulation.  Q : c128 [ncomp*nsrc][nk] (columns are
+ * eigenvectors, already scaled as wanted); out : c64 [nk][M].              */
+int oi_kernels_from_gram(const void* G, const double* w, int M, int nsrc,
+                         int ncomp, const void* Q, int nk, void* out,
+                         void* stream);
+
+/* Hermitian eigensolver, float64 (parallel cyclic Jacobi).  Replaces the
+ * torch.svd of DecomposeTCC_SOCS.py:7 (for a Hermitian PSD TCC the singular
+ * values and left vectors are |eigenvalues| and eigenvectors).
+ *   A : c128 [n][n] Hermitian, overwritten (diagonalised)
+ *   w : float64 [n] eigenvalues (unsorted, in V's column order)
+ *   V : c128 [n][n] eigenvectors as columns
+ *   work : oi_eigh_work_bytes(n) bytes; sweeps_out : host int or NULL.
+ * Synchronises the stream once per sweep (convergence test).              */
+size_t oi_eigh_work_bytes(int n);
+int oi_eigh_hermitian(void* A, int n, double* w, void* V, int max_sweeps, double tol,
+                      void* work, int* sweeps_out, void* stream);
+
+/* -------------------------------------------------------------- elementwise
+ * Fringe Zernike wavefront W(rho, theta) (Projection.py:13-128, Orientation
+ * already applied to coeffs).  rho, theta, out : float32 [n].              */
+int oi_zernike(const float* rho, const float* theta, int n, const double* coeffs,
+               float* out, void* stream);
+
+/* Characteristic matrix (CalculateCharacteristicMatrix.py:3-16).
+ * f, g : float32 [n]; out : c64 [6][n] = Mxx, Myx, Mxy, Myy, Mxz, Myz.      */
+int oi_char_matrix(const float* f, const float* g, int n, double na,
+                   double index_image, void* out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OPTICALIMAGING_AMD_H */

@@ -1,0 +1,109 @@
+"""Aerial-image entry points on the HIP device (CalculateAerialImage.py).
+
+``CalculateAbbeImage``: partially coherent Abbe source-point summation
+(CalculateAerialImage.py:18-103).  Each lit source point s and polarisation
+component e is one coherent kernel sqrt(n V_s / W) P_se(kappa) on the
+spectrum window (``oi_pupil_kernels``, float64, rho^2 <= 1 decided in
+float64); the image is the coherent sum over kernels and field channels
+(``oi_coherent_image``), evaluated on a coarse grid sized by one shifted
+pupil (2 floor(2 p) + 1) and Fourier-interpolated exactly to the image grid.
+
+``CalculateHopkinsImage``: per focus, TCC -> SOCS kernels -> SOCS image
+(CalculateAerialImage.py:105-130).
+"""
+from __future__ import annotations
+
+import torch
+
+from . import _native as nat
+from . import engine, optics
+from .CalculateAerialImage_SOCS import CalculateAerialImage_SOCS
+from .CalculateTCCMatrix import CalculateTCCMatrix
+from .DecomposeTCC_SOCS import DecomposeTCC_SOCS
+from .Scatter import _pitches, pupil_image
+
+__all__ = ["ImageData", "cartesian_to_polar", "CalculateAbbeImage", "CalculateHopkinsImage",
+           "abbe_images"]
+
+
+class ImageData:
+    """Result record (CalculateAerialImage.py:7-11)."""
+    Intensity: torch.Tensor = None
+    ImageX: torch.Tensor = None
+    ImageY: torch.Tensor = None
+    ImageZ: torch.Tensor = None
+
+
+def cartesian_to_polar(x, y):
+    """CalculateAerialImage.py:13-16."""
+    return torch.sqrt(x ** 2 + y ** 2), torch.atan2(y, x)
+
+
+def abbe_images(field, source, scatter, projector, numerics) -> torch.Tensor:
+    """Abbe intensities (nFocus, Ny, Nx) float32 on the device."""
+    vector = optics.imaging_mode(numerics)
+    dev = nat.device()
+    h = torch.as_tensor(field)
+    ny, nx = int(h.shape[0]), int(h.shape[1])
+    px, py = _pitches(scatter, projector, source)
+    src = optics.source_set(source)
+    k0r, wr, k0c, wc, br, bc = optics.abbe_window(src, py, px, ny, nx)
+    tens, kind, cin, live = engine.field_encoding(h)
+    plan = engine.PLANS.get(ny=ny, nx=nx, k0r=k0r, wr=wr, k0c=k0c, wc=wc, nchan=len(live),
+                            max_batch=1, band_r=br, band_c=bc)
+    lib = nat.load()
+    spec = torch.empty((1, len(live), wr, wc), dtype=torch.complex64, device=dev)
+    cptr, keep = nat.int_array(live)
+    nat.check(lib.oi_spectrum_window(plan.handle, nat.ptr(tens), kind, cin, cptr, 1, nat.ptr(spec),
+                                     nat.stream()))
+    del keep
+    scale = torch.sqrt(float(projector.IndexImage) * src.V / src.weight)
+    foci = optics.focus_values(projector)
+    out = torch.empty((len(foci), ny, nx), dtype=torch.float32, device=dev)
+    for i, z in enumerate(foci):
+        opt = optics.optics_record(source, projector, z, py, px, vector)
+        G = optics.pupil_kernels(src, opt, scale, k0r, wr, k0c, wc)
+        nat.check(lib.oi_coherent_image(plan.handle, nat.ptr(spec), 1, nat.ptr(G),
+                                        G.shape[0] * G.shape[1], nat.ptr(out[i]), nat.stream()))
+    return out
+
+
+def _image_axes(scatter, nx, ny, projector):
+    d = ImageData()
+    d.ImageX = torch.linspace(-scatter.Period_X / 2, scatter.Period_X / 2, nx)
+    d.ImageY = torch.linspace(-scatter.Period_Y / 2, scatter.Period_Y / 2, ny)
+    d.ImageZ = projector.FocusRange
+    return d
+
+
+def CalculateAbbeImage(source, scatter, projector, numerics):
+    """Returns (ImageData, PupilImage); Intensity is (nFocus, Ny, Nx) float32
+    on the ScatterField's device (CalculateAerialImage.py:18-103)."""
+    h = torch.as_tensor(scatter.ScatterField)
+    ny, nx = int(h.shape[0]), int(h.shape[1])
+    intensity = abbe_images(h, source, scatter, projector, numerics)
+    px, py = _pitches(scatter, projector, source)
+    pupil = pupil_image(h, px, py)
+    d = _image_axes(scatter, nx, ny, projector)
+    d.Intensity = intensity.to(h.device)
+    return d, pupil.to(h.device)
+
+
+def CalculateHopkinsImage(source, scatter, projector, numerics):
+    """Returns (ImageData, PupilImage) (CalculateAerialImage.py:105-130)."""
+    h = torch.as_tensor(scatter.ScatterField)
+    ny, nx = int(h.shape[0]), int(h.shape[1])
+    pitchxy = [scatter.Period_Y, scatter.Period_X]
+    nfg = [nx, ny]
+    foci = optics.focus_values(projector)
+    intensity = torch.zeros((len(foci), ny, nx))
+    pupil = None
+    for i, z in enumerate(foci):
+        tcc, fg = CalculateTCCMatrix(source, pitchxy, projector, z, numerics)
+        kern = DecomposeTCC_SOCS(tcc, fg, nfg, numerics)
+        img, pupil = CalculateAerialImage_SOCS(scatter, kern, source, projector)
+        intensity[i] = img.cpu()
+        del tcc
+    d = _image_axes(scatter, nx, ny, projector)
+    d.Intensity = intensity.to(h.device)
+    return d, pupil

@@ -1,0 +1,403 @@
+// C ABI: error state, twiddle tables, imaging plans and the imaging entry
+// points (spectrum window, coherent image, SOCS image, full spectrum).
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "oi_internal.h"
+
+namespace oi {
+
+static thread_local std::string g_last_error;
+
+void set_error(int code, const std::string& msg) {
+  (void)code;
+  g_last_error = msg;
+}
+
+int fail(int code, const std::string& msg) {
+  set_error(code, msg);
+  return code;
+}
+
+int make_twiddles(const int* sizes, int count, FftDesc* descs, TwiddlePool& pool) {
+  std::vector<int> uniq;
+  for (int i = 0; i < count; ++i) {
+    if (!fft_factor(sizes[i], descs[i]))
+      return fail(OI_ERR_UNSUPPORTED, "FFT length " + std::to_string(sizes[i]) +
+                                          " is not a product of 2, 3 and 5");
+    bool seen = false;
+    for (int u : uniq) seen |= (u == sizes[i]);
+    if (!seen) uniq.push_back(sizes[i]);
+  }
+  size_t total = 0;
+  for (int u : uniq) total += (size_t)u;
+  std::vector<float2> host(total);
+  std::vector<size_t> off(uniq.size());
+  size_t pos = 0;
+  for (size_t k = 0; k < uniq.size(); ++k) {
+    off[k] = pos;
+    const int n = uniq[k];
+    for (int m = 0; m < n; ++m) {
+      // exp(-2 pi i m / n) in float64, rounded once.
+      const double ang = 2.0 * M_PI * (double)m / (double)n;
+      host[pos + m] = make_float2((float)std::cos(ang), (float)-std::sin(ang));
+    }
+    pos += (size_t)n;
+  }
+  float2* dev = nullptr;
+  if (hipMalloc(&dev, total * sizeof(float2)) != hipSuccess)
+    return fail(OI_ERR_NOMEM, "twiddle table allocation failed");
+  if (hipMemcpy(dev, host.data(), total * sizeof(float2), hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipFree(dev);
+    return fail(OI_ERR_RUNTIME, "twiddle table upload failed");
+  }
+  for (int i = 0; i < count; ++i)
+    for (size_t k = 0; k < uniq.size(); ++k)
+      if (uniq[k] == sizes[i]) descs[i].tw = dev + off[k];
+  pool.dev = dev;
+  pool.bytes = total * sizeof(float2);
+  return OI_OK;
+}
+
+void free_twiddles(TwiddlePool& pool) {
+  if (pool.dev) (void)hipFree(pool.dev);
+  pool.dev = nullptr;
+  pool.bytes = 0;
+}
+
+static bool smooth235(int n) {
+  if (n < 1) return false;
+  for (int p : {2, 3, 5})
+    while (n % p == 0) n /= p;
+  return n == 1;
+}
+
+// Smallest 2,3,5-smooth multiple of 8 that is >= need; n when that reaches n.
+static int choose_coarse(int need, int n) {
+  if (need >= n) return n;
+  int L = need < 8 ? 8 : need;
+  while (!(smooth235(L) && L % 8 == 0)) ++L;
+  return L >= n ? n : L;
+}
+
+}  // namespace oi
+
+using namespace oi;
+
+// Stage boundaries recorded per call when timing is on: before spectrum rows,
+// after rows, after columns, after the coherent sum, after the interpolation
+// columns, after the interpolation rows.
+constexpr int kStages = 5;
+
+struct oi_plan {
+  oi_geometry g;
+  FftDesc fx, fy, flr, flc, frows;
+  TwiddlePool pool;
+  float2* T = nullptr;
+  float2* spec = nullptr;
+  float2* R = nullptr;
+  float2* U = nullptr;
+  float* Ic = nullptr;
+  size_t bytes = 0;
+  std::vector<hipEvent_t> ev;  // slots * (kStages + 1)
+  int slots = 0, next = 0, recorded = 0;
+};
+
+static void mark(oi_plan* p, int stage, hipStream_t st) {
+  if (p->slots == 0) return;
+  (void)hipEventRecord(p->ev[(size_t)p->next * (kStages + 1) + stage], st);
+}
+
+static void mark_done(oi_plan* p) {
+  if (p->slots == 0) return;
+  p->next = (p->next + 1) % p->slots;
+  if (p->recorded < p->slots) ++p->recorded;
+}
+
+extern "C" const char* oi_last_error(void) { return g_last_error.c_str(); }
+
+extern "C" int oi_version(void) { return 10000; }
+
+static void plan_free(oi_plan* p) {
+  if (!p) return;
+  for (hipEvent_t e : p->ev) (void)hipEventDestroy(e);
+  for (void* ptr : {(void*)p->T, (void*)p->spec, (void*)p->R, (void*)p->U, (void*)p->Ic})
+    if (ptr) (void)hipFree(ptr);
+  free_twiddles(p->pool);
+  delete p;
+}
+
+extern "C" int oi_plan_create(oi_geometry* geo, oi_plan** out) {
+  if (!geo || !out) return fail(OI_ERR_INVALID, "plan_create: null argument");
+  *out = nullptr;
+  oi_geometry g = *geo;
+  if (g.ny <= 0 || g.nx <= 0 || g.wr <= 0 || g.wc <= 0 || g.nchan <= 0 || g.nchan > 3 ||
+      g.max_batch <= 0)
+    return fail(OI_ERR_INVALID, "plan_create: sizes must be positive (nchan <= 3)");
+  if (g.wr > g.ny || g.wc > g.nx)
+    return fail(OI_ERR_INVALID, "plan_create: spectrum window larger than the grid");
+  if (g.k0r < -(g.ny / 2) || g.k0r + g.wr - 1 > g.ny - 1 - g.ny / 2 || g.k0c < -(g.nx / 2) ||
+      g.k0c + g.wc - 1 > g.nx - 1 - g.nx / 2)
+    return fail(OI_ERR_INVALID, "plan_create: window outside the centred index range");
+  if (!fft_size_ok(g.ny) || !fft_size_ok(g.nx))
+    return fail(OI_ERR_UNSUPPORTED, "plan_create: grid sizes must be products of 2, 3 and 5");
+  const int band_r = g.band_r > 0 ? g.band_r : g.wr;
+  const int band_c = g.band_c > 0 ? g.band_c : g.wc;
+  g.band_r = band_r;
+  g.band_c = band_c;
+  int lr = g.lr > 0 ? g.lr : choose_coarse(2 * band_r - 1, g.ny);
+  int lc = g.lc > 0 ? g.lc : choose_coarse(2 * band_c - 1, g.nx);
+  if (lr > g.ny || lc > g.nx) return fail(OI_ERR_INVALID, "plan_create: coarse grid exceeds the image grid");
+  // A coarse grid equal to the image grid computes the image directly (this
+  // also covers bands wider than the grid, where the image itself aliases).
+  g.direct = (lr == g.ny || lc == g.nx) ? 1 : 0;
+  if (!g.direct && (lr < 2 * band_r - 1 || lc < 2 * band_c - 1))
+    return fail(OI_ERR_INVALID, "plan_create: coarse grid smaller than 2*band-1");
+  if (g.direct) {
+    lr = g.ny;
+    lc = g.nx;
+  }
+  if (!fft_size_ok(lr) || !fft_size_ok(lc))
+    return fail(OI_ERR_INVALID, "plan_create: coarse sizes must be products of 2, 3 and 5");
+  g.lr = lr;
+  g.lc = lc;
+  g.qr = g.direct ? 0 : band_r - 1;
+  g.qc = g.direct ? 0 : band_c - 1;
+  // Coarse rows per workgroup: largest of 16, 8, 4, 2, 1 dividing lr whose
+  // accumulator and shared-memory footprint fit.
+  int rows = 0;
+  for (int cand : {16, 8, 4, 2, 1}) {
+    if (lr % cand) continue;
+    if ((size_t)cand * lc > (size_t)kThreads * kAccMax) continue;
+    if (coherent_lds_bytes(g.wc, cand, lc) > 120 * 1024) continue;
+    rows = cand;
+    break;
+  }
+  if (!rows) return fail(OI_ERR_UNSUPPORTED, "plan_create: no workgroup shape fits this geometry");
+  g.rows = rows;
+  // Kernel groups: fixed per plan so an image's summation order never depends
+  // on the batch it is computed in.  0 = enough groups for ~512 workgroups.
+  if (g.max_groups <= 0) {
+    const int phases = lr / rows;
+    int gr = (512 + phases * g.max_batch - 1) / (phases * g.max_batch);
+    g.max_groups = gr < 1 ? 1 : (gr > 64 ? 64 : gr);
+  }
+
+  oi_plan* p = new (std::nothrow) oi_plan();
+  if (!p) return fail(OI_ERR_NOMEM, "plan_create: host allocation failed");
+  p->g = g;
+  int sizes[5] = {g.nx, g.ny, lr, lc, rows};
+  FftDesc d[5];
+  int rc = make_twiddles(sizes, 5, d, p->pool);
+  if (rc) {
+    plan_free(p);
+    return rc;
+  }
+  p->fx = d[0];
+  p->fy = d[1];
+  p->flr = d[2];
+  p->flc = d[3];
+  p->frows = d[4];
+  const size_t B = (size_t)g.max_batch, C = (size_t)g.nchan, G = (size_t)g.max_groups;
+  struct Want {
+    void** ptr;
+    size_t bytes;
+  } want[] = {
+      {(void**)&p->T, B * C * g.wc * g.ny * sizeof(float2)},
+      {(void**)&p->spec, B * C * g.wr * g.wc * sizeof(float2)},
+      {(void**)&p->R, g.direct ? 0 : B * G * lr * (size_t)(g.qc + 1) * sizeof(float2)},
+      {(void**)&p->U, g.direct ? 0 : B * (size_t)g.ny * (g.qc + 1) * sizeof(float2)},
+      {(void**)&p->Ic, (g.direct && G > 1) ? B * G * (size_t)lr * lc * sizeof(float) : 0},
+  };
+  p->bytes = p->pool.bytes;
+  for (auto& w : want) {
+    if (!w.bytes) continue;
+    if (hipMalloc(w.ptr, w.bytes) != hipSuccess) {
+      plan_free(p);
+      return fail(OI_ERR_NOMEM, "plan_create: scratch allocation failed");
+    }
+    p->bytes += w.bytes;
+  }
+  *geo = g;
+  *out = p;
+  return OI_OK;
+}
+
+extern "C" int oi_plan_destroy(oi_plan* plan) {
+  plan_free(plan);
+  return OI_OK;
+}
+
+extern "C" size_t oi_plan_bytes(const oi_plan* plan) { return plan ? plan->bytes : 0; }
+
+static int check_field(const oi_plan* p, const void* field, int kind, int cin, const int* chans,
+                       int batch) {
+  if (!p || !field) return fail(OI_ERR_INVALID, "null plan or field");
+  if (batch < 1 || batch > p->g.max_batch) return fail(OI_ERR_INVALID, "batch outside [1, max_batch]");
+  if (kind == OI_FIELD_REAL) {
+    if (p->g.nchan != 1) return fail(OI_ERR_INVALID, "real masks carry one channel");
+  } else if (kind == OI_FIELD_C64_CHLAST) {
+    if (!chans || cin < 1) return fail(OI_ERR_INVALID, "complex fields need cin and chans");
+    for (int i = 0; i < p->g.nchan; ++i)
+      if (chans[i] < 0 || chans[i] >= cin) return fail(OI_ERR_INVALID, "channel index out of range");
+  } else {
+    return fail(OI_ERR_INVALID, "unknown field kind");
+  }
+  return OI_OK;
+}
+
+extern "C" int oi_spectrum_window(oi_plan* p, const void* field, int kind, int cin,
+                                  const int* chans, int batch, void* spec, void* stream) {
+  int rc = check_field(p, field, kind, cin, chans, batch);
+  if (rc) return rc;
+  if (!spec) return fail(OI_ERR_INVALID, "null spec");
+  hipStream_t st = (hipStream_t)stream;
+  const oi_geometry& g = p->g;
+  mark(p, 0, st);
+  rc = launch_spec_rows(field, kind, cin, chans, g.nchan, batch, g.ny, g.nx, p->fx, g.k0c, g.wc,
+                        p->T, st);
+  if (rc) return rc;
+  mark(p, 1, st);
+  rc = launch_spec_cols(p->T, batch * g.nchan, g.ny, g.nx, p->fy, g.k0r, g.wr, g.wc,
+                        static_cast<float2*>(spec), st);
+  mark(p, 2, st);
+  return rc;
+}
+
+extern "C" int oi_coherent_image(oi_plan* p, const void* spec, int batch, const void* kern,
+                                 int nkern, float* out, void* stream) {
+  if (!p || !spec || !kern || !out) return fail(OI_ERR_INVALID, "coherent_image: null argument");
+  if (batch < 1 || batch > p->g.max_batch) return fail(OI_ERR_INVALID, "batch outside [1, max_batch]");
+  if (nkern < 1) return fail(OI_ERR_INVALID, "coherent_image: nkern must be >= 1");
+  hipStream_t st = (hipStream_t)stream;
+  const oi_geometry& g = p->g;
+  const int phases = g.lr / g.rows;
+  int groups = g.max_groups < nkern ? g.max_groups : nkern;
+  CohArgs a{};
+  a.spec = static_cast<const float2*>(spec);
+  a.kern = static_cast<const float2*>(kern);
+  a.nchan = g.nchan;
+  a.nkern = nkern;
+  a.k0r = g.k0r;
+  a.wr = g.wr;
+  a.k0c = g.k0c;
+  a.wc = g.wc;
+  a.lr = g.lr;
+  a.lc = g.lc;
+  a.rows = g.rows;
+  a.phases = phases;
+  a.groups = groups;
+  a.qc = g.qc;
+  a.direct = g.direct;
+  a.frows = p->frows;
+  a.flc = p->flc;
+  a.tw_lr = p->flr.tw;
+  a.R = p->R;
+  a.Ic = g.direct ? (groups == 1 ? out : p->Ic) : nullptr;
+  int rc = launch_coherent(a, batch, st);
+  if (rc) return rc;
+  mark(p, 3, st);
+  if (g.direct) {
+    if (groups > 1) rc = launch_group_sum(p->Ic, batch, groups, (size_t)g.ny * g.nx, out, st);
+    mark(p, 4, st);
+    mark(p, 5, st);
+    mark_done(p);
+    return rc;
+  }
+  InterpArgs ia{};
+  ia.R = p->R;
+  ia.groups = groups;
+  ia.lr = g.lr;
+  ia.lc = g.lc;
+  ia.ny = g.ny;
+  ia.nx = g.nx;
+  ia.qr = g.qr;
+  ia.qc = g.qc;
+  ia.scale = 1.0f / ((float)g.lr * (float)g.lc);
+  ia.flr = p->flr;
+  ia.fny = p->fy;
+  ia.fnx = p->fx;
+  ia.U = p->U;
+  ia.out = out;
+  struct Ctx {
+    oi_plan* p;
+    hipStream_t st;
+  } ctx{p, st};
+  rc = launch_interp(ia, batch, st, [](void* c, int stage) {
+    Ctx* x = static_cast<Ctx*>(c);
+    mark(x->p, stage, x->st);
+  }, &ctx);
+  mark_done(p);
+  return rc;
+}
+
+extern "C" int oi_plan_set_timing(oi_plan* p, int slots) {
+  if (!p || slots < 0) return fail(OI_ERR_INVALID, "plan_set_timing: bad arguments");
+  for (hipEvent_t e : p->ev) (void)hipEventDestroy(e);
+  p->ev.clear();
+  p->slots = 0;
+  p->next = 0;
+  p->recorded = 0;
+  if (slots == 0) return OI_OK;
+  p->ev.resize((size_t)slots * (kStages + 1));
+  for (auto& e : p->ev) OI_HIP_CHECK(hipEventCreate(&e));
+  p->slots = slots;
+  return OI_OK;
+}
+
+extern "C" int oi_plan_read_timing(oi_plan* p, double* stage_ms, int* ncalls) {
+  if (!p || !stage_ms || !ncalls) return fail(OI_ERR_INVALID, "plan_read_timing: bad arguments");
+  for (int k = 0; k < kStages; ++k) stage_ms[k] = 0.0;
+  *ncalls = p->recorded;
+  for (int c = 0; c < p->recorded; ++c) {
+    hipEvent_t* e = &p->ev[(size_t)c * (kStages + 1)];
+    OI_HIP_CHECK(hipEventSynchronize(e[kStages]));
+    for (int k = 0; k < kStages; ++k) {
+      float ms = 0.f;
+      OI_HIP_CHECK(hipEventElapsedTime(&ms, e[k], e[k + 1]));
+      stage_ms[k] += ms;
+    }
+  }
+  return OI_OK;
+}
+
+extern "C" int oi_socs_image(oi_plan* p, const void* field, int kind, int cin, const int* chans,
+                             int batch, const void* kern, int nkern, float* out, void* stream) {
+  int rc = oi_spectrum_window(p, field, kind, cin, chans, batch, p ? p->spec : nullptr, stream);
+  if (rc) return rc;
+  return oi_coherent_image(p, p->spec, batch, kern, nkern, out, stream);
+}
+
+extern "C" int oi_spectrum_full(const void* field, int kind, int cin, const int* chans, int nchan,
+                                int batch, int ny, int nx, float scale, void* work, void* hk,
+                                void* stream) {
+  if (!field || !work || !hk || batch < 1 || ny < 1 || nx < 1 || nchan < 1 || nchan > 3)
+    return fail(OI_ERR_INVALID, "spectrum_full: bad arguments");
+  if (kind == OI_FIELD_REAL) {
+    if (nchan != 1) return fail(OI_ERR_INVALID, "spectrum_full: real fields carry one channel");
+  } else if (kind == OI_FIELD_C64_CHLAST) {
+    if (!chans) return fail(OI_ERR_INVALID, "spectrum_full: chans required");
+    for (int i = 0; i < nchan; ++i)
+      if (chans[i] < 0 || chans[i] >= cin) return fail(OI_ERR_INVALID, "spectrum_full: bad channel");
+  } else {
+    return fail(OI_ERR_INVALID, "spectrum_full: unknown field kind");
+  }
+  int sizes[2] = {nx, ny};
+  FftDesc d[2];
+  TwiddlePool pool;
+  int rc = make_twiddles(sizes, 2, d, pool);
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  const float s = scale / ((float)ny * (float)nx);
+  rc = launch_full_rows(field, kind, cin, chans, nchan, batch, ny, nx, d[0],
+                        static_cast<float2*>(work), st);
+  if (!rc)
+    rc = launch_full_cols(static_cast<const float2*>(work), nchan, batch, ny, nx, d[1], s,
+                          static_cast<float2*>(hk), st);
+  // The tables must outlive the kernels that read them.
+  if (hipStreamSynchronize(st) != hipSuccess && !rc) rc = fail(OI_ERR_RUNTIME, "spectrum_full: sync failed");
+  free_twiddles(pool);
+  return rc;
+}

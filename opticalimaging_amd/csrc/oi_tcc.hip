@@ -1,0 +1,400 @@
+// Hermitian TCC contraction on MFMA, and the float64 Gram route.
+//
+// TCC[m][m'] = alpha sum_r a_r[m] conj(a_r[m']),  a_r = sqrt(w_s) G[e][s] (r = (e, s)).
+// With a = x + i y over K = 2r real rows X[m] = (x, y), Z[m] = (-y, x):
+//   Re TCC = X X^T,  Im TCC = X Z^T,
+// two real GEMMs over K = 2r.  Operands are stored [M][Kp] (K fastest) so a
+// lane's MFMA fragment is one 16-byte load.  Only tiles J >= I are computed; an
+// off-diagonal tile is also written transposed and conjugated.
+//
+// Split-bf16 (3 terms): x = hi + lo with hi = bf16(x), lo = bf16(x - hi);
+// x x' ~ hi hi' + hi lo' + lo hi' accumulated in fp32 (the lo lo' term, ~2^-16
+// relative, is dropped).
+#include <cmath>
+
+#include "oi_internal.h"
+
+namespace oi {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kTile = 128;  // output tile per workgroup (4 waves of 64 x 64)
+
+__device__ __forceinline__ unsigned short bf16_bits(float x) {
+  // round-to-nearest-even; inputs here are finite
+  unsigned int u = __float_as_uint(x);
+  const unsigned int lsb = (u >> 16) & 1u;
+  u += 0x7fffu + lsb;
+  return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ float bf16_value(unsigned short h) { return __uint_as_float((unsigned int)h << 16); }
+
+// Operand preparation.  mode 0: bf16 hi/lo planes [4][M][Kp] = Xh, Xl, Zh, Zl;
+// mode 1: fp32 planes [2][M][Kp] = X, Z; mode 2: fp64 planes [2][M][Kp].
+__global__ void k_tcc_prep(const float2* __restrict__ G, const float* __restrict__ w, int M, int nsrc,
+                           int ncomp, int Kp, int mode, void* __restrict__ work) {
+  const int r = ncomp * nsrc;
+  const size_t total = (size_t)M * Kp;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (size_t)gridDim.x * blockDim.x) {
+    const int m = (int)(e / Kp), k = (int)(e - (size_t)m * Kp);
+    float xv = 0.f, zv = 0.f;
+    if (k < 2 * r) {
+      const int rr = k < r ? k : k - r;
+      const int s = rr % nsrc;
+      const float2 a = G[(size_t)rr * M + m];
+      const float sw = sqrtf(w[s]);
+      const float re = a.x * sw, im = a.y * sw;
+      if (k < r) {
+        xv = re;
+        zv = -im;
+      } else {
+        xv = im;
+        zv = re;
+      }
+    }
+    if (mode == 0) {
+      unsigned short* P = static_cast<unsigned short*>(work);
+      const unsigned short xh = bf16_bits(xv), zh = bf16_bits(zv);
+      P[e] = xh;
+      P[total + e] = bf16_bits(xv - bf16_value(xh));
+      P[2 * total + e] = zh;
+      P[3 * total + e] = bf16_bits(zv - bf16_value(zh));
+    } else if (mode == 1) {
+      float* P = static_cast<float*>(work);
+      P[e] = xv;
+      P[total + e] = zv;
+    } else {
+      double* P = static_cast<double*>(work);
+      P[e] = (double)xv;
+      P[total + e] = (double)zv;
+    }
+  }
+}
+
+__device__ __forceinline__ void tile_pair(int t, int nt, int& I, int& J) {
+  // t enumerates pairs I <= J row by row.
+  I = 0;
+  int rem = t;
+  while (rem >= nt - I) {
+    rem -= nt - I;
+    ++I;
+  }
+  J = I + rem;
+}
+
+template <typename T, int V>
+__device__ __forceinline__ void load_frag(const T* __restrict__ P, int M, int Kp, int row, int k,
+                                          T* out) {
+  if (row < M) {
+#pragma unroll
+    for (int v = 0; v < V; ++v) out[v] = P[(size_t)row * Kp + k + v];
+  } else {
+#pragma unroll
+    for (int v = 0; v < V; ++v) out[v] = T(0);
+  }
+}
+
+// Writes a 16 x 16 accumulator fragment and, for off-diagonal tiles, its
+// conjugate transpose.  Lane l holds column l&15 and rows 4(l>>4)+i for the
+// fp32-accumulating forms, rows (l>>4)+4i for the fp64 form (INTERLEAVED).
+template <bool INTERLEAVED, typename A>
+__device__ __forceinline__ void store_frag(float2* __restrict__ tcc, int M, int r0, int c0,
+                                           const A& re, const A& im, double alpha, bool mirror) {
+  const int lane = threadIdx.x & 63;
+  const int col = c0 + (lane & 15);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int row = r0 + (INTERLEAVED ? (lane >> 4) + 4 * i : (lane >> 4) * 4 + i);
+    if (row < M && col < M) {
+      const float vr = (float)(alpha * (double)re[i]);
+      const float vi = (float)(alpha * (double)im[i]);
+      tcc[(size_t)row * M + col] = make_float2(vr, vi);
+      if (mirror) tcc[(size_t)col * M + row] = make_float2(vr, -vi);
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_tcc_bf16(const unsigned short* __restrict__ planes, int M,
+                                                  int Kp, int nt, double alpha,
+                                                  float2* __restrict__ tcc) {
+  const size_t plane = (size_t)M * Kp;
+  const __bf16* Xh = reinterpret_cast<const __bf16*>(planes);
+  const __bf16* Xl = Xh + plane;
+  const __bf16* Zh = Xh + 2 * plane;
+  const __bf16* Zl = Xh + 3 * plane;
+  int I, J;
+  tile_pair(blockIdx.x, nt, I, J);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int r0 = I * kTile + (wave >> 1) * 64;
+  const int c0 = J * kTile + (wave & 1) * 64;
+  const int li = lane & 15, lk = (lane >> 4) * 8;
+  f32x4 accR[4][4], accI[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      accR[a][b] = f32x4{0, 0, 0, 0};
+      accI[a][b] = f32x4{0, 0, 0, 0};
+    }
+  for (int pass = 0; pass < 3; ++pass) {
+    const __bf16* As = pass == 2 ? Xl : Xh;
+    const __bf16* Br = pass == 1 ? Xl : Xh;
+    const __bf16* Bi = pass == 1 ? Zl : Zh;
+    for (int k0 = 0; k0 < Kp; k0 += 32) {
+      bf16x8 fa[4], fr[4], fi[4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const int ra = r0 + t * 16 + li, cb = c0 + t * 16 + li;
+        const size_t oa = (size_t)ra * Kp + k0 + lk, ob = (size_t)cb * Kp + k0 + lk;
+        fa[t] = ra < M ? *reinterpret_cast<const bf16x8*>(As + oa) : bf16x8{};
+        fr[t] = cb < M ? *reinterpret_cast<const bf16x8*>(Br + ob) : bf16x8{};
+        fi[t] = cb < M ? *reinterpret_cast<const bf16x8*>(Bi + ob) : bf16x8{};
+      }
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+          accR[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[a], fr[b], accR[a][b], 0, 0, 0);
+          accI[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[a], fi[b], accI[a][b], 0, 0, 0);
+        }
+    }
+  }
+  const bool mirror = I != J;
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+      store_frag<false>(tcc, M, r0 + a * 16, c0 + b * 16, accR[a][b], accI[a][b], alpha, mirror);
+}
+
+// fp32 MFMA 16x16x4: lane fragment is one value at k = l>>4; a 16-wide K step
+// runs four MFMAs, sub-step kk using k = k0 + 4 (l>>4) + kk for both operands.
+__global__ __launch_bounds__(256) void k_tcc_f32(const float* __restrict__ planes, int M, int Kp,
+                                                 int nt, double alpha, float2* __restrict__ tcc) {
+  const size_t plane = (size_t)M * Kp;
+  const float* X = planes;
+  const float* Z = planes + plane;
+  int I, J;
+  tile_pair(blockIdx.x, nt, I, J);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int r0 = I * kTile + (wave >> 1) * 64;
+  const int c0 = J * kTile + (wave & 1) * 64;
+  const int li = lane & 15, lk = (lane >> 4) * 4;
+  f32x4 accR[4][4], accI[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      accR[a][b] = f32x4{0, 0, 0, 0};
+      accI[a][b] = f32x4{0, 0, 0, 0};
+    }
+  for (int k0 = 0; k0 < Kp; k0 += 16) {
+    float fa[4][4], fr[4][4], fi[4][4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      load_frag<float, 4>(X, M, Kp, r0 + t * 16 + li, k0 + lk, fa[t]);
+      load_frag<float, 4>(X, M, Kp, c0 + t * 16 + li, k0 + lk, fr[t]);
+      load_frag<float, 4>(Z, M, Kp, c0 + t * 16 + li, k0 + lk, fi[t]);
+    }
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk)
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+          accR[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[a][kk], fr[b][kk], accR[a][b], 0, 0, 0);
+          accI[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[a][kk], fi[b][kk], accI[a][b], 0, 0, 0);
+        }
+  }
+  const bool mirror = I != J;
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+      store_frag<false>(tcc, M, r0 + a * 16, c0 + b * 16, accR[a][b], accI[a][b], alpha, mirror);
+}
+
+// fp64 MFMA 16x16x4 (64 x 32 per wave to bound registers: tile 128 x 64 per block).
+__global__ __launch_bounds__(256) void k_tcc_f64(const double* __restrict__ planes, int M, int Kp,
+                                                 int nt, double alpha, float2* __restrict__ tcc) {
+  const size_t plane = (size_t)M * Kp;
+  const double* X = planes;
+  const double* Z = planes + plane;
+  int I, J;
+  tile_pair(blockIdx.x, nt, I, J);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int r0 = I * kTile + (wave >> 1) * 64;
+  const int li = lane & 15, lk = (lane >> 4) * 2;
+  for (int half = 0; half < 2; ++half) {
+    const int c0 = J * kTile + (wave & 1) * 64 + half * 32;
+    f64x4 accR[4][2], accI[4][2];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        accR[a][b] = f64x4{0, 0, 0, 0};
+        accI[a][b] = f64x4{0, 0, 0, 0};
+      }
+    for (int k0 = 0; k0 < Kp; k0 += 8) {
+      double fa[4][2], fr[2][2], fi[2][2];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) load_frag<double, 2>(X, M, Kp, r0 + t * 16 + li, k0 + lk, fa[t]);
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        load_frag<double, 2>(X, M, Kp, c0 + t * 16 + li, k0 + lk, fr[t]);
+        load_frag<double, 2>(Z, M, Kp, c0 + t * 16 + li, k0 + lk, fi[t]);
+      }
+#pragma unroll
+      for (int kk = 0; kk < 2; ++kk)
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+          for (int b = 0; b < 2; ++b) {
+            accR[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[a][kk], fr[b][kk], accR[a][b], 0, 0, 0);
+            accI[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[a][kk], fi[b][kk], accI[a][b], 0, 0, 0);
+          }
+    }
+    const bool mirror = I != J;
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+        store_frag<true>(tcc, M, r0 + a * 16, c0 + b * 16, accR[a][b], accI[a][b], alpha, mirror);
+  }
+}
+
+// H[r][r'] = sqrt(w_s w_s') sum_m conj(G[r][m]) G[r'][m], float64.  Block =
+// 16 x 16 entries (upper blocks only, mirrored), m streamed through LDS.
+__global__ __launch_bounds__(256) void k_gram(const float2* __restrict__ G, const double* __restrict__ w,
+                                              int M, int nsrc, int n, int nb,
+                                              double2* __restrict__ H) {
+  __shared__ double2 sa[16][65];
+  __shared__ double2 sb[16][65];
+  int I, J;
+  tile_pair(blockIdx.x, nb, I, J);
+  const int ti = threadIdx.x >> 4, tj = threadIdx.x & 15;
+  const int ra = I * 16, rb = J * 16;
+  double accr = 0.0, acci = 0.0;
+  for (int m0 = 0; m0 < M; m0 += 64) {
+    for (int e = threadIdx.x; e < 16 * 64; e += 256) {
+      const int rr = e >> 6, mm = e & 63;
+      const int m = m0 + mm;
+      float2 va = make_float2(0.f, 0.f), vb = make_float2(0.f, 0.f);
+      if (m < M) {
+        if (ra + rr < n) va = G[(size_t)(ra + rr) * M + m];
+        if (rb + rr < n) vb = G[(size_t)(rb + rr) * M + m];
+      }
+      sa[rr][mm] = make_double2(va.x, va.y);
+      sb[rr][mm] = make_double2(vb.x, vb.y);
+    }
+    __syncthreads();
+#pragma unroll 8
+    for (int mm = 0; mm < 64; ++mm) {
+      const double2 a = sa[ti][mm], b = sb[tj][mm];
+      accr += a.x * b.x + a.y * b.y;  // conj(a) b
+      acci += a.x * b.y - a.y * b.x;
+    }
+    __syncthreads();
+  }
+  const int r = ra + ti, c = rb + tj;
+  if (r < n && c < n) {
+    const double s = sqrt(w[r % nsrc] * w[c % nsrc]);
+    H[(size_t)r * n + c] = make_double2(s * accr, s * acci);
+    if (I != J) H[(size_t)c * n + r] = make_double2(s * accr, -s * acci);
+  }
+}
+
+// out[k][m] = sum_r sqrt(w_s) G[r][m] Q[r][k], float64 accumulation.
+__global__ void k_from_gram(const float2* __restrict__ G, const double* __restrict__ w, int M, int nsrc,
+                            int n, const double2* __restrict__ Q, int nk, float2* __restrict__ out) {
+  const size_t total = (size_t)nk * M;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (size_t)gridDim.x * blockDim.x) {
+    const int k = (int)(e / M), m = (int)(e - (size_t)k * M);
+    double ar = 0.0, ai = 0.0;
+    for (int r = 0; r < n; ++r) {
+      const float2 g = G[(size_t)r * M + m];
+      const double2 q = Q[(size_t)r * nk + k];
+      const double s = sqrt(w[r % nsrc]);
+      ar += s * (g.x * q.x - g.y * q.y);
+      ai += s * (g.x * q.y + g.y * q.x);
+    }
+    out[e] = make_float2((float)ar, (float)ai);
+  }
+}
+
+static size_t kp_for(int nsrc, int ncomp, int precision) {
+  const size_t k2 = 2ull * nsrc * ncomp;
+  const size_t q = precision == OI_TCC_SPLIT_BF16 ? 32 : (precision == OI_TCC_FP32 ? 16 : 8);
+  return (k2 + q - 1) / q * q;
+}
+
+}  // namespace oi
+
+using namespace oi;
+
+extern "C" size_t oi_tcc_work_bytes(int M, int nsrc, int ncomp, int precision) {
+  if (M < 1 || nsrc < 1 || ncomp < 1) return 0;
+  const size_t kp = kp_for(nsrc, ncomp, precision);
+  const size_t e = (size_t)M * kp;
+  if (precision == OI_TCC_SPLIT_BF16) return 4 * e * sizeof(unsigned short);
+  if (precision == OI_TCC_FP32) return 2 * e * sizeof(float);
+  return 2 * e * sizeof(double);
+}
+
+extern "C" int oi_tcc_build(const void* G, const float* w, int M, int nsrc, int ncomp, double alpha,
+                            int precision, void* work, void* tcc, void* stream) {
+  if (!G || !w || !work || !tcc || M < 1 || nsrc < 1 || ncomp < 1)
+    return fail(OI_ERR_INVALID, "tcc_build: bad arguments");
+  if (precision != OI_TCC_SPLIT_BF16 && precision != OI_TCC_FP32 && precision != OI_TCC_FP64)
+    return fail(OI_ERR_INVALID, "tcc_build: unknown precision");
+  hipStream_t st = (hipStream_t)stream;
+  const int Kp = (int)kp_for(nsrc, ncomp, precision);
+  const int mode = precision == OI_TCC_SPLIT_BF16 ? 0 : (precision == OI_TCC_FP32 ? 1 : 2);
+  size_t tot = (size_t)M * Kp;
+  int blocks = (int)((tot + 255) / 256 > 16384 ? 16384 : (tot + 255) / 256);
+  hipLaunchKernelGGL(k_tcc_prep, dim3(blocks), dim3(256), 0, st, static_cast<const float2*>(G), w, M,
+                     nsrc, ncomp, Kp, mode, work);
+  OI_LAUNCH_CHECK("tcc_prep");
+  const int nt = (M + kTile - 1) / kTile;
+  const int pairs = nt * (nt + 1) / 2;
+  float2* out = static_cast<float2*>(tcc);
+  if (mode == 0)
+    hipLaunchKernelGGL(k_tcc_bf16, dim3(pairs), dim3(256), 0, st,
+                       static_cast<const unsigned short*>(work), M, Kp, nt, alpha, out);
+  else if (mode == 1)
+    hipLaunchKernelGGL(k_tcc_f32, dim3(pairs), dim3(256), 0, st, static_cast<const float*>(work), M,
+                       Kp, nt, alpha, out);
+  else
+    hipLaunchKernelGGL(k_tcc_f64, dim3(pairs), dim3(256), 0, st, static_cast<const double*>(work), M,
+                       Kp, nt, alpha, out);
+  OI_LAUNCH_CHECK("tcc_gemm");
+  return OI_OK;
+}
+
+extern "C" int oi_tcc_gram(const void* G, const double* w, int M, int nsrc, int ncomp, void* H,
+                           void* stream) {
+  if (!G || !w || !H || M < 1 || nsrc < 1 || ncomp < 1) return fail(OI_ERR_INVALID, "tcc_gram: bad arguments");
+  const int n = nsrc * ncomp;
+  const int nb = (n + 15) / 16;
+  hipLaunchKernelGGL(k_gram, dim3(nb * (nb + 1) / 2), dim3(256), 0, (hipStream_t)stream,
+                     static_cast<const float2*>(G), w, M, nsrc, n, nb, static_cast<double2*>(H));
+  OI_LAUNCH_CHECK("tcc_gram");
+  return OI_OK;
+}
+
+extern "C" int oi_kernels_from_gram(const void* G, const double* w, int M, int nsrc, int ncomp,
+                                    const void* Q, int nk, void* out, void* stream) {
+  if (!G || !w || !Q || !out || M < 1 || nsrc < 1 || ncomp < 1 || nk < 1)
+    return fail(OI_ERR_INVALID, "kernels_from_gram: bad arguments");
+  const size_t total = (size_t)nk * M;
+  int blocks = (int)((total + 255) / 256 > 16384 ? 16384 : (total + 255) / 256);
+  hipLaunchKernelGGL(k_from_gram, dim3(blocks), dim3(256), 0, (hipStream_t)stream,
+                     static_cast<const float2*>(G), w, M, nsrc, nsrc * ncomp,
+                     static_cast<const double2*>(Q), nk, static_cast<float2*>(out));
+  OI_LAUNCH_CHECK("kernels_from_gram");
+  return OI_OK;
+}

@@ -1,0 +1,208 @@
+"""Device-side engine: imaging plans, SOCS kernel sets and batched imaging.
+
+A ``KernelSet`` is the device-resident, windowed form of SOCS kernels: the
+c64 array [K][wr][wc] holding each kernel at spectrum index kappa =
+(k0r + i, k0c + j), on a grid of ny x nx samples.  It is what
+``DecomposeTCC_SOCS`` produces (in dense form, ``DecomposeTCC_SOCS.py:24-35``)
+restricted to the kernels' support, so imaging never touches the dense
+(ny, nx, K) array.
+
+``Imager`` owns one library plan per geometry and runs
+``oi_socs_image`` / ``oi_coherent_image`` on the current stream.
+"""
+from __future__ import annotations
+
+import collections
+import ctypes
+from dataclasses import dataclass
+
+import torch
+
+from . import _native as nat
+
+
+def centred(idx: torch.Tensor, n: int) -> torch.Tensor:
+    """Centred spectrum index of array positions (DC-at-0 layout)."""
+    return torch.remainder(idx + n // 2, n) - n // 2
+
+
+@dataclass
+class KernelSet:
+    """Windowed SOCS kernels on the device."""
+    win: torch.Tensor      # c64 [K][wr][wc], cuda
+    k0r: int
+    k0c: int
+    ny: int
+    nx: int
+
+    @property
+    def K(self) -> int:
+        return int(self.win.shape[0])
+
+    @property
+    def wr(self) -> int:
+        return int(self.win.shape[1])
+
+    @property
+    def wc(self) -> int:
+        return int(self.win.shape[2])
+
+    @staticmethod
+    def from_dense(dense: torch.Tensor) -> "KernelSet":
+        """Window of dense kernels (ny, nx, K) in the layout of
+        DecomposeTCC_SOCS.py:31 (fftshifted embedding), found from their exact
+        non-zero support."""
+        dev = nat.device()
+        lib = nat.load()
+        d = dense.to(device=dev, dtype=torch.complex64).contiguous()
+        if d.dim() != 3:
+            raise ValueError("SOCS kernels must be a (Ny, Nx, K) tensor")
+        ny, nx, K = d.shape
+        rowflag = torch.zeros(ny, dtype=torch.int32, device=dev)
+        colflag = torch.zeros(nx, dtype=torch.int32, device=dev)
+        nat.check(lib.oi_kernel_support(nat.ptr(d), ny, nx, K, nat.ptr(rowflag),
+                                        nat.ptr(colflag), nat.stream()))
+        rows = torch.nonzero(rowflag).reshape(-1)
+        cols = torch.nonzero(colflag).reshape(-1)
+        if rows.numel() == 0:
+            return KernelSet(torch.zeros((K, 1, 1), dtype=torch.complex64, device=dev), 0, 0, ny, nx)
+        kr = centred(rows, ny)
+        kc = centred(cols, nx)
+        k0r, k1r = int(kr.min()), int(kr.max())
+        k0c, k1c = int(kc.min()), int(kc.max())
+        wr, wc = k1r - k0r + 1, k1c - k0c + 1
+        win = torch.empty((K, wr, wc), dtype=torch.complex64, device=dev)
+        nat.check(lib.oi_kernel_window(nat.ptr(d), ny, nx, K, k0r, wr, k0c, wc, nat.ptr(win),
+                                       nat.stream()))
+        return KernelSet(win, k0r, k0c, ny, nx)
+
+    def dense(self) -> torch.Tensor:
+        """Dense (ny, nx, K) c64 form (DecomposeTCC_SOCS.py:24-35 layout)."""
+        dev = self.win.device
+        out = torch.empty((self.ny, self.nx, self.K), dtype=torch.complex64, device=dev)
+        nat.check(nat.load().oi_kernel_dense(nat.ptr(self.win), self.K, self.k0r, self.wr,
+                                             self.k0c, self.wc, self.ny, self.nx, nat.ptr(out),
+                                             nat.stream()))
+        return out
+
+
+class Plan:
+    """One oi_plan (twiddles + scratch) for a fixed geometry."""
+
+    def __init__(self, ny, nx, k0r, wr, k0c, wc, nchan=1, max_batch=1, max_groups=0,
+                 band_r=0, band_c=0, lr=0, lc=0):
+        lib = nat.load()
+        g = nat.Geometry(ny=ny, nx=nx, k0r=k0r, wr=wr, k0c=k0c, wc=wc, band_r=band_r,
+                         band_c=band_c, nchan=nchan, max_batch=max_batch, max_groups=max_groups,
+                         lr=lr, lc=lc)
+        h = ctypes.c_void_p()
+        nat.check(lib.oi_plan_create(ctypes.byref(g), ctypes.byref(h)))
+        self.handle = h
+        self.geo = g
+        self.key = (ny, nx, k0r, wr, k0c, wc, nchan, max_batch, max_groups, band_r, band_c, lr, lc)
+
+    @property
+    def bytes(self) -> int:
+        return int(nat.load().oi_plan_bytes(self.handle))
+
+    def info(self) -> dict:
+        return {n: getattr(self.geo, n) for n, _ in nat.Geometry._fields_}
+
+    STAGES = ("spectrum_rows", "spectrum_cols", "coherent", "interp_cols", "interp_rows")
+
+    def set_timing(self, slots: int) -> None:
+        """Record HIP events between the five launches of the next `slots`
+        imaging calls (0 turns recording off)."""
+        nat.check(nat.load().oi_plan_set_timing(self.handle, int(slots)))
+
+    def read_timing(self):
+        """(per-stage summed ms dict, number of recorded calls)."""
+        ms = (ctypes.c_double * 5)()
+        n = ctypes.c_int(0)
+        nat.check(nat.load().oi_plan_read_timing(self.handle, ms, ctypes.byref(n)))
+        return dict(zip(self.STAGES, list(ms))), int(n.value)
+
+    def close(self):
+        if self.handle:
+            torch.cuda.synchronize()
+            nat.load().oi_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            if self.handle and torch.cuda.is_available():
+                self.close()
+        except Exception:
+            pass
+
+
+class _PlanCache:
+    def __init__(self, capacity=8):
+        self.capacity = capacity
+        self.plans = collections.OrderedDict()
+
+    def get(self, **kw) -> Plan:
+        key = tuple(sorted(kw.items()))
+        p = self.plans.get(key)
+        if p is not None:
+            self.plans.move_to_end(key)
+            return p
+        p = Plan(**kw)
+        self.plans[key] = p
+        while len(self.plans) > self.capacity:
+            _, old = self.plans.popitem(last=False)
+            old.close()
+        return p
+
+
+PLANS = _PlanCache()
+
+
+def field_encoding(field):
+    """Choose the cheapest device encoding of a ScatterField (Ny, Nx, C), or a
+    batch of them (B, Ny, Nx, C): returns (tensor, kind, cin, live channels).
+    Channels that are identically zero contribute nothing to any image
+    (CalculateAerialImage.py:76-87, CalculateAerialImage_SOCS.py:5-9) and are
+    skipped; a single live channel with zero imaginary part is carried as a
+    float32 mask [B, Ny, Nx]."""
+    dev = nat.device()
+    f = torch.as_tensor(field).to(device=dev)
+    if f.dim() not in (3, 4):
+        raise ValueError("ScatterField must be (Ny, Nx, C) or (B, Ny, Nx, C)")
+    if f.dim() == 3:
+        f = f.unsqueeze(0)
+    cin = f.shape[-1]
+    live = [c for c in range(cin) if bool((f[..., c] != 0).any())]
+    if not live:
+        live = [0]
+    if len(live) == 1 and not torch.is_complex(f):
+        return (f[..., live[0]].to(torch.float32).contiguous(), nat.OI_FIELD_REAL, 1, [0])
+    if len(live) == 1 and torch.is_complex(f) and bool((f[..., live[0]].imag == 0).all()):
+        return (f[..., live[0]].real.to(torch.float32).contiguous(), nat.OI_FIELD_REAL, 1, [0])
+    return f.to(torch.complex64).contiguous(), nat.OI_FIELD_C64_CHLAST, cin, live
+
+
+def socs_image(masks: torch.Tensor, ks: KernelSet, kind: int = nat.OI_FIELD_REAL, cin: int = 1,
+               chans=(0,), out: torch.Tensor | None = None, max_batch: int | None = None,
+               plan: Plan | None = None) -> torch.Tensor:
+    """Batched SOCS intensity (CalculateAerialImage_SOCS.py:3-11 per mask).
+
+    masks: float32 [B, ny, nx] (kind REAL) or c64 [B, ny, nx, cin] on the
+    device.  Returns float32 [B, ny, nx] in the reference's output layout
+    (after its final fftshift)."""
+    B = int(masks.shape[0])
+    ny, nx = int(masks.shape[1]), int(masks.shape[2])
+    if (ny, nx) != (ks.ny, ks.nx):
+        raise ValueError(f"mask grid {ny}x{nx} does not match kernel grid {ks.ny}x{ks.nx}")
+    chans = list(chans)
+    nchan = len(chans)
+    if plan is None:
+        plan = PLANS.get(ny=ny, nx=nx, k0r=ks.k0r, wr=ks.wr, k0c=ks.k0c, wc=ks.wc, nchan=nchan,
+                         max_batch=max(B, max_batch or 0))
+    if out is None:
+        out = torch.empty((B, ny, nx), dtype=torch.float32, device=masks.device)
+    cptr, keep = nat.int_array(chans)
+    nat.check(nat.load().oi_socs_image(plan.handle, nat.ptr(masks), kind, cin, cptr, B,
+                                       nat.ptr(ks.win), ks.K, nat.ptr(out), nat.stream()))
+    del keep
+    return out

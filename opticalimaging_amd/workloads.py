@@ -75,7 +75,7 @@ def manhattan(n, period, rng, cd_range=(45.0, 90.0), lines=None):
         elif kind == 1:     # horizontal 1D line
             m |= _wrap_range(n, pitch, start, cd, period)[:, None]
         else:               # 2D segment
-            length = rng.uniform(4 * cd, period / 2)
+            length = rng.uniform(2 * cd, max(4 * cd, period / 2))
             other = rng.uniform(0, period)
             if rng.integers(0, 2):
                 m |= (_wrap_range(n, pitch, other, length, period)[:, None]

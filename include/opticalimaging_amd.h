@@ -72,7 +72,9 @@ int oi_version(void);
  * optionally lr, lc (coarse grid; 0 = choose).
  * Outputs filled by the library: max_groups (when 0), lr, lc, rows (coarse rows per workgroup),
  * direct (1 when the coarse grid equals the image grid), qr, qc (half-widths
- * of the intensity band carried from the coarse to the image grid).
+ * of the intensity band carried from the coarse to the image grid), path
+ * (1 when the image sizes and a square coarse grid have compile-time-size
+ * kernels; then lr = lc).
  *
  * The coarse grid must satisfy lr >= 2*band_r - 1 where band_r is the widest
  * row extent of any single coherent kernel's support (and likewise for
@@ -89,6 +91,7 @@ typedef struct oi_geometry {
   int rows;
   int direct;
   int qr, qc;
+  int path;       /* 1: compile-time-size kernels, 0: generic kernels      */
 } oi_geometry;
 
 typedef struct oi_plan oi_plan;

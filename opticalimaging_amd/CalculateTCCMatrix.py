@@ -38,6 +38,7 @@ class TCCFactor:
     alpha: float
     lenf: int
     leng: int
+    pitch: tuple = (0.0, 0.0)   # (p_0, p_1) = xypitch * NA / lambda
 
 
 def precision_code(numerics=None) -> int:
@@ -86,7 +87,7 @@ def CalculateTCCMatrix(source, xypitch, projector, focus, numerics):
     G = G.reshape(ncomp, src.n, lenf * leng)
     alpha = float(projector.IndexImage) / src.weight
     tcc = tcc_contract(G, src.V, alpha, precision_code(numerics))
-    tcc._oi_factor = TCCFactor(G=G, V=src.V, alpha=alpha, lenf=lenf, leng=leng)
+    tcc._oi_factor = TCCFactor(G=G, V=src.V, alpha=alpha, lenf=lenf, leng=leng, pitch=(p0, p1))
     return tcc, [leng, lenf]
 
 

@@ -9,13 +9,15 @@ eigenvector u = B q / sqrt(l), so each SOCS kernel u sqrt(l) = B q
 The float64 Hermitian eigensolver is this library's parallel Jacobi
 (``oi_eigh_hermitian``) on the device.
 
-The returned dense kernels (Ny, Nx, K) complex64 follow the reference layout
+On the factor route K stops at the TCC's rank (further kernels are exactly
+zero).  The returned dense kernels (Ny, Nx, K) complex64 follow the reference layout
 (embedding at rows (Ny-len f+1)//2, columns (Nx-len g+1)//2, then fftshift);
 the tensor carries the device-resident windowed kernel set as ``_oi_kernels``.
 """
 from __future__ import annotations
 
 import ctypes
+import math
 
 import torch
 
@@ -70,7 +72,8 @@ def eigen_desc(H: torch.Tensor):
 
 
 def _window_kernels(tcc, M, numerics):
-    """(K, M) c64 kernels u_k sqrt(s_k) in TCC-grid order and K."""
+    """(K, M) c64 kernels u_k sqrt(s_k) in TCC-grid order, K, and the pupil
+    band when the kernels hold every non-zero eigenpair (else None)."""
     dev = tcc.device
     lib = nat.load()
     factor = getattr(tcc, "_oi_factor", None)
@@ -87,17 +90,26 @@ def _window_kernels(tcc, M, numerics):
         s_all[:r] = s[:r]
         K = socs_count(s_all, M, numerics)
         keff = min(K, r)
-        win = torch.zeros((max(K, 1), M), dtype=torch.complex64, device=dev)
+        win = torch.zeros((max(keff, 1), M), dtype=torch.complex64, device=dev)
         if keff > 0:
             Qk = Q[:, :keff].contiguous()
             nat.check(lib.oi_kernels_from_gram(nat.ptr(G), nat.ptr(w), M, nsrc, ncomp, nat.ptr(Qk),
                                                keff, nat.ptr(win), nat.stream()))
-        return win[:K], K
+        band = None
+        if K >= n and factor.pitch[0] > 0:
+            # every eigenpair kept: sum_k phi_k phi_k^H is the TCC, whose image
+            # band is one pupil diameter, 2 floor(2 p) per axis
+            band = (int(math.floor(2 * factor.pitch[0])) + 1, int(math.floor(2 * factor.pitch[1])) + 1)
+        # Kernels past the TCC's rank are exactly zero and change no image; they
+        # are not materialised (with K = M - 1 on a large grid the reference's
+        # dense (Ny, Nx, K) array would not fit in memory).
+        K = max(keff, 1)
+        return win[:K], K, band
     T = tcc.to(torch.complex128)
     s, U = eigen_desc(T)
     K = socs_count(s, M, numerics)
     win = (U[:, :K] * torch.sqrt(s[:K])[None, :]).transpose(0, 1).to(torch.complex64).contiguous()
-    return win, K
+    return win, K, None
 
 
 def DecomposeTCC_SOCS(TCCMatrix_Stacked, FG_ValidSize, Nfg, numerics):
@@ -116,15 +128,18 @@ def DecomposeTCC_SOCS(TCCMatrix_Stacked, FG_ValidSize, Nfg, numerics):
     if tcc_dev.shape != (M, M):
         raise ValueError("TCC size does not match FG_ValidSize")
     if lenf > ny or leng > nx:
-        raise ValueError("TCC grid larger than the mask grid")
-    win, K = _window_kernels(tcc_dev, M, numerics)
+        # the reference fails here with torch's RuntimeError (DecomposeTCC_SOCS.py:28-30)
+        raise RuntimeError("TCC grid larger than the mask grid")
+    win, K, band = _window_kernels(tcc_dev, M, numerics)
     win = win.reshape(K, lenf, leng)
     r0 = (ny - lenf + 1) // 2
     c0 = (nx - leng + 1) // 2
     temp = torch.zeros((K, ny, nx), dtype=torch.complex64, device=dev)
     temp[:, r0:r0 + lenf, c0:c0 + leng] = win
     dense = torch.fft.fftshift(temp, dim=(1, 2)).permute(1, 2, 0).contiguous()
-    dense._oi_kernels = _kernel_set(win, r0, c0, ny, nx, dense)
+    ks = _kernel_set(win, r0, c0, ny, nx, dense)
+    ks.band = band
+    dense._oi_kernels = ks
     return dense
 
 

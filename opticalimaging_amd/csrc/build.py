@@ -22,8 +22,8 @@ LIB_NAME = "libopticalimaging_amd.so"
 LIB_PATH = os.path.join(PKG, LIB_NAME)
 ARCH = os.environ.get("OI_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["oi_api.hip", "oi_imaging.hip", "oi_pupil.hip", "oi_tcc.hip", "oi_eigh.hip",
-           "oi_adjoint.hip"]
+SOURCES = ["oi_api.hip", "oi_imaging.hip", "oi_fastpath.hip", "oi_pupil.hip", "oi_tcc.hip",
+           "oi_eigh.hip", "oi_adjoint.hip"]
 
 COMMON_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
@@ -56,7 +56,8 @@ def build(verbose: bool = False) -> str:
         with concurrent.futures.ThreadPoolExecutor(workers) as ex:
             list(ex.map(_compile, srcs, objs))
         tmp_lib = os.path.join(tmp, LIB_NAME)
-        cmd = [_hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp_lib]
+        cmd = [_hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-Wl,--no-undefined", *objs,
+               "-o", tmp_lib]
         res = subprocess.run(cmd, capture_output=True, text=True)
         if res.returncode != 0:
             raise RuntimeError(f"link failed:\n{res.stderr}")

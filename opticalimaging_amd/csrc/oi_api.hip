@@ -5,7 +5,7 @@
 #include <new>
 #include <vector>
 
-#include "oi_internal.h"
+#include "oi_fastpath.h"
 
 namespace oi {
 
@@ -100,6 +100,7 @@ struct oi_plan {
   float2* R = nullptr;
   float2* U = nullptr;
   float* Ic = nullptr;
+  float2* Ihat = nullptr;
   size_t bytes = 0;
   std::vector<hipEvent_t> ev;  // slots * (kStages + 1)
   int slots = 0, next = 0, recorded = 0;
@@ -123,7 +124,8 @@ extern "C" int oi_version(void) { return 10000; }
 static void plan_free(oi_plan* p) {
   if (!p) return;
   for (hipEvent_t e : p->ev) (void)hipEventDestroy(e);
-  for (void* ptr : {(void*)p->T, (void*)p->spec, (void*)p->R, (void*)p->U, (void*)p->Ic})
+  for (void* ptr : {(void*)p->T, (void*)p->spec, (void*)p->R, (void*)p->U, (void*)p->Ic,
+                    (void*)p->Ihat})
     if (ptr) (void)hipFree(ptr);
   free_twiddles(p->pool);
   delete p;
@@ -165,10 +167,23 @@ extern "C" int oi_plan_create(oi_geometry* geo, oi_plan** out) {
   g.lc = lc;
   g.qr = g.direct ? 0 : band_r - 1;
   g.qc = g.direct ? 0 : band_c - 1;
+  // Compile-time-size path: both image sizes tabled and a tabled square coarse
+  // grid L >= 2 band - 1 in both directions, smaller than the image.
+  g.path = 0;
+  if (!g.direct && geo->lr <= 0 && geo->lc <= 0 && fp_image_size_ok(g.ny) && fp_image_size_ok(g.nx)) {
+    const int need = (2 * band_r - 1) > (2 * band_c - 1) ? 2 * band_r - 1 : 2 * band_c - 1;
+    const int L = fp_coarse_choose(need);
+    if (L > 0 && L < g.ny && L < g.nx && fp_coherent_rows_fit(L) &&
+        fp_coherent_lds(L, g.wr, g.wc) <= kLdsLimit) {
+      g.path = 1;
+      g.lr = g.lc = lr = lc = L;
+    }
+  }
   // Coarse rows per workgroup: largest of 16, 8, 4, 2, 1 dividing lr whose
   // accumulator and shared-memory footprint fit.
-  int rows = 0;
+  int rows = g.path ? lr : 0;
   for (int cand : {16, 8, 4, 2, 1}) {
+    if (g.path) break;
     if (lr % cand) continue;
     if ((size_t)cand * lc > (size_t)kThreads * kAccMax) continue;
     if (coherent_lds_bytes(g.wc, cand, lc) > 120 * 1024) continue;
@@ -176,11 +191,11 @@ extern "C" int oi_plan_create(oi_geometry* geo, oi_plan** out) {
     break;
   }
   if (!rows) return fail(OI_ERR_UNSUPPORTED, "plan_create: no workgroup shape fits this geometry");
-  g.rows = rows;
+  g.rows = rows;  // the compile-time path keeps every coarse row in one block
   // Kernel groups: fixed per plan so an image's summation order never depends
   // on the batch it is computed in.  0 = enough groups for ~512 workgroups.
   if (g.max_groups <= 0) {
-    const int phases = lr / rows;
+    const int phases = lr / g.rows;
     int gr = (512 + phases * g.max_batch - 1) / (phases * g.max_batch);
     g.max_groups = gr < 1 ? 1 : (gr > 64 ? 64 : gr);
   }
@@ -210,6 +225,7 @@ extern "C" int oi_plan_create(oi_geometry* geo, oi_plan** out) {
       {(void**)&p->R, g.direct ? 0 : B * G * lr * (size_t)(g.qc + 1) * sizeof(float2)},
       {(void**)&p->U, g.direct ? 0 : B * (size_t)g.ny * (g.qc + 1) * sizeof(float2)},
       {(void**)&p->Ic, (g.direct && G > 1) ? B * G * (size_t)lr * lc * sizeof(float) : 0},
+      {(void**)&p->Ihat, g.path ? B * (size_t)(g.qc + 1) * (2 * g.qr + 1) * sizeof(float2) : 0},
   };
   p->bytes = p->pool.bytes;
   for (auto& w : want) {
@@ -248,6 +264,48 @@ static int check_field(const oi_plan* p, const void* field, int kind, int cin, c
   return OI_OK;
 }
 
+static void mark_cb(void* ctx, int stage);
+
+struct MarkCtx {
+  oi_plan* p;
+  hipStream_t st;
+};
+
+static thread_local MarkCtx g_mark_ctx;
+
+static void mark_cb(void* ctx, int stage) {
+  MarkCtx* x = static_cast<MarkCtx*>(ctx);
+  mark(x->p, stage, x->st);
+}
+
+static FpArgs fast_args(oi_plan* p, int batch, hipStream_t st) {
+  const oi_geometry& g = p->g;
+  FpArgs a{};
+  a.nchan = g.nchan;
+  a.batch = batch;
+  a.ny = g.ny;
+  a.nx = g.nx;
+  a.k0r = g.k0r;
+  a.wr = g.wr;
+  a.k0c = g.k0c;
+  a.wc = g.wc;
+  a.L = g.lr;
+  a.qr = g.qr;
+  a.qc = g.qc;
+  a.tw_x = p->fx.tw;
+  a.tw_y = p->fy.tw;
+  a.tw_l = p->flr.tw;
+  a.T = p->T;
+  a.spec = p->spec;
+  a.R = p->R;
+  a.Ihat = p->Ihat;
+  a.U = p->U;
+  g_mark_ctx = MarkCtx{p, st};
+  a.mark = mark_cb;
+  a.mark_ctx = &g_mark_ctx;
+  return a;
+}
+
 extern "C" int oi_spectrum_window(oi_plan* p, const void* field, int kind, int cin,
                                   const int* chans, int batch, void* spec, void* stream) {
   int rc = check_field(p, field, kind, cin, chans, batch);
@@ -256,6 +314,17 @@ extern "C" int oi_spectrum_window(oi_plan* p, const void* field, int kind, int c
   hipStream_t st = (hipStream_t)stream;
   const oi_geometry& g = p->g;
   mark(p, 0, st);
+  if (g.path) {
+    FpArgs a = fast_args(p, batch, st);
+    a.field = field;
+    a.kind = kind;
+    a.cin = cin;
+    for (int i = 0; i < g.nchan; ++i) a.chans.c[i] = kind == OI_FIELD_REAL ? 0 : chans[i];
+    a.spec = static_cast<float2*>(spec);
+    rc = fp_spectrum(a, st);
+    mark(p, 2, st);
+    return rc;
+  }
   rc = launch_spec_rows(field, kind, cin, chans, g.nchan, batch, g.ny, g.nx, p->fx, g.k0c, g.wc,
                         p->T, st);
   if (rc) return rc;
@@ -275,6 +344,27 @@ extern "C" int oi_coherent_image(oi_plan* p, const void* spec, int batch, const 
   const oi_geometry& g = p->g;
   const int phases = g.lr / g.rows;
   int groups = g.max_groups < nkern ? g.max_groups : nkern;
+  if (g.path) {
+    FpArgs a = fast_args(p, batch, st);
+    a.out = out;
+    FpCoh c{};
+    c.spec = static_cast<const float2*>(spec);
+    c.kern = static_cast<const float2*>(kern);
+    c.nchan = g.nchan;
+    c.nkern = nkern;
+    c.k0r = g.k0r;
+    c.wr = g.wr;
+    c.k0c = g.k0c;
+    c.wc = g.wc;
+    c.groups = groups;
+    c.qc = g.qc;
+    c.tw_l = p->flr.tw;
+    c.tw_m = p->flr.tw;
+    c.R = p->R;
+    int rc = fp_image(a, c, st);
+    mark_done(p);
+    return rc;
+  }
   CohArgs a{};
   a.spec = static_cast<const float2*>(spec);
   a.kern = static_cast<const float2*>(kern);

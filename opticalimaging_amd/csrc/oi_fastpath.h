@@ -1,0 +1,54 @@
+// Fast-path (compile-time transform sizes) imaging launches.
+#pragma once
+
+#include "oi_internal.h"
+
+namespace oi {
+
+struct Chans {
+  int c[4];
+};
+
+// Shared launch arguments of the fast path.
+struct FpArgs {
+  const void* field;
+  int kind, cin, nchan;
+  Chans chans;
+  int batch, ny, nx;
+  int k0r, wr, k0c, wc;
+  int L;          // square coarse grid
+  int qr, qc;     // band half-widths
+  const float2* tw_x;  // exp(-2 pi i m / nx)
+  const float2* tw_y;  // exp(-2 pi i m / ny)
+  const float2* tw_l;  // exp(-2 pi i m / L)
+  float2* T;      // [B][C][ny][wc]
+  float2* spec;   // [B][C][wr][wc]
+  float2* R;      // [B][G][L][qc+1]
+  float2* Ihat;   // [B][qc+1][2qr+1]
+  float2* U;      // [B][qc+1][ny]
+  float* out;     // [B][ny][nx]
+  StageMark mark;
+  void* mark_ctx;
+};
+
+struct FpCoh {
+  const float2* spec;
+  const float2* kern;
+  int nchan, nkern;
+  int k0r, wr, k0c, wc;
+  int groups;
+  int qc;
+  const float2* tw_l;  // exp(-2 pi i m / L) (stage twiddles and phase twist)
+  const float2* tw_m;  // column-stage transform table
+  float2* R;
+};
+
+bool fp_image_size_ok(int n);
+bool fp_coarse_size_ok(int l);
+int fp_coarse_choose(int need);          // smallest table size >= need, 0 if none
+size_t fp_coherent_lds(int L, int wr, int wc);
+int fp_coherent_rows_fit(int L);
+int fp_spectrum(const FpArgs& a, hipStream_t st);  // marks stage 1 between its launches
+int fp_image(const FpArgs& a, const FpCoh& c, hipStream_t st);  // marks stages 3, 4, 5
+
+}  // namespace oi

@@ -1,0 +1,599 @@
+// Imaging kernels with compile-time transform sizes (register FFTs, one LDS
+// exchange per transform).  Same mathematics and data flow as the generic
+// kernels of oi_imaging.hip; used when the image sizes and the coarse grid are
+// in the size tables below.
+//
+//   k_fp_rows_real<P>   mask rows -> T[img][r][j] = sum_m h[r][m] e^{+2 pi i (k0c+j) m / nx}
+//   k_fp_rows_c64<P>    same for one channel of a complex channels-last field
+//   k_fp_cols<P>        T columns -> spec[img][c][i][j] (window rows), scaled 1/(ny nx)
+//   k_fp_coherent<PL,PM> spectrum x kernels -> coarse |E|^2 -> row DFT -> R
+//   k_fp_band<PL>       R columns -> band of the coarse intensity spectrum Ihat
+//   k_fp_ucols<P>       Ihat columns zero-padded to ny -> U[img][q][y]
+//   k_fp_urows<P>       U rows (Hermitian) -> image rows (C2R)
+#include <type_traits>
+
+#include "oi_fastpath.h"
+#include "oi_fftreg.h"
+
+namespace oi {
+namespace fp {
+
+using reg::GFFT;
+using reg::pad;
+
+constexpr int kBlock = 256;
+
+template <class P>
+constexpr int groups_per_block() {
+  return kBlock / P::T;
+}
+
+// Transform scratch is allocated for every thread of the block, including the
+// threads of an incomplete last group (they run the transforms' barriers with
+// zeros and must not write into a live group's scratch).
+template <class P>
+constexpr int scratch_groups() {
+  return (kBlock + P::T - 1) / P::T;
+}
+
+// ------------------------------------------------------------------ rows
+template <class P>
+__global__ __launch_bounds__(kBlock) void k_fp_rows_real(const float* __restrict__ mask, int ny,
+                                                         int k0c, int wc,
+                                                         const float2* __restrict__ twt,
+                                                         float2* __restrict__ T) {
+  constexpr int N = P::N, TT = P::T, V = P::V, G = groups_per_block<P>();
+  extern __shared__ float2 lds[];
+  const int g = threadIdx.x / TT, t = threadIdx.x - g * TT;
+  const bool live = g < G;
+  const int gg = live ? g : 0;
+  float2* scr = lds + g * P::LDS;
+  const int img = blockIdx.y;
+  const int r0 = 2 * (blockIdx.x * G + gg), r1 = r0 + 1;
+  typename P::Tw tw;
+  P::load_tw(tw, t, twt);
+  const float* src = mask + (size_t)img * ny * N;
+  float2 v[V];
+#pragma unroll
+  for (int m = 0; m < V; ++m) {
+    const int x = t + TT * m;
+    const float a = (live && r0 < ny) ? src[(size_t)r0 * N + x] : 0.f;
+    const float b = (live && r1 < ny) ? src[(size_t)r1 * N + x] : 0.f;
+    v[m] = make_float2(a, b);
+  }
+  P::template run<true>(v, scr, t, tw);
+  if (live) {
+#pragma unroll
+    for (int m = 0; m < V; ++m) scr[pad(t + TT * m)] = v[m];
+  }
+  __syncthreads();
+  if (!live || r0 >= ny) return;
+  float2* out0 = T + ((size_t)img * ny + r0) * wc;
+  for (int j = t; j < wc; j += TT) {
+    const int k = pmod(k0c + j, N);
+    const int km = k == 0 ? 0 : N - k;
+    const float2 zk = scr[pad(k)], zm = scr[pad(km)];
+    out0[j] = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));            // (Z[k] + conj Z[-k]) / 2
+    if (r1 < ny) out0[wc + j] = make_float2(0.5f * (zk.y + zm.y), -0.5f * (zk.x - zm.x));  // / 2i
+  }
+}
+
+template <class P>
+__global__ __launch_bounds__(kBlock) void k_fp_rows_c64(const float2* __restrict__ field, int cin,
+                                                        Chans ch, int nchan, int ny, int k0c,
+                                                        int wc, const float2* __restrict__ twt,
+                                                        float2* __restrict__ T) {
+  constexpr int N = P::N, TT = P::T, V = P::V, G = groups_per_block<P>();
+  extern __shared__ float2 lds[];
+  const int g = threadIdx.x / TT, t = threadIdx.x - g * TT;
+  const bool live = g < G;
+  const int gg = live ? g : 0;
+  float2* scr = lds + g * P::LDS;
+  const int bc = blockIdx.y;
+  const int img = bc / nchan, ci = bc - img * nchan;
+  const int chan = ch.c[ci];
+  const int r = blockIdx.x * G + gg;
+  typename P::Tw tw;
+  P::load_tw(tw, t, twt);
+  float2 v[V];
+#pragma unroll
+  for (int m = 0; m < V; ++m) {
+    const int x = t + TT * m;
+    v[m] = (live && r < ny) ? field[(((size_t)img * ny + r) * N + x) * cin + chan] : make_float2(0.f, 0.f);
+  }
+  P::template run<true>(v, scr, t, tw);
+  if (live) {
+#pragma unroll
+    for (int m = 0; m < V; ++m) scr[pad(t + TT * m)] = v[m];
+  }
+  __syncthreads();
+  if (!live || r >= ny) return;
+  float2* out = T + ((size_t)bc * ny + r) * wc;
+  for (int j = t; j < wc; j += TT) out[j] = scr[pad(pmod(k0c + j, N))];
+}
+
+// --------------------------------------------------------------- columns
+template <class P>
+__global__ __launch_bounds__(kBlock) void k_fp_cols(const float2* __restrict__ T, int k0r, int wr,
+                                                    int wc, float scale,
+                                                    const float2* __restrict__ twt,
+                                                    float2* __restrict__ spec) {
+  constexpr int N = P::N, TT = P::T, V = P::V, G = groups_per_block<P>();
+  extern __shared__ float2 lds[];
+  const int g = threadIdx.x / TT, t = threadIdx.x - g * TT;
+  const bool live = g < G;
+  const int gg = live ? g : 0;
+  const int bc = blockIdx.y;
+  const int j = blockIdx.x * G + gg;
+  const bool col = live && j < wc;
+  typename P::Tw tw;
+  P::load_tw(tw, t, twt);
+  const float2* src = T + (size_t)bc * N * wc;
+  float2 v[V];
+#pragma unroll
+  for (int m = 0; m < V; ++m) v[m] = col ? src[(size_t)(t + TT * m) * wc + j] : make_float2(0.f, 0.f);
+  P::template run<true>(v, lds + g * P::LDS, t, tw);
+  if (!col) return;
+  float2* dst = spec + (size_t)bc * wr * wc;
+#pragma unroll
+  for (int m = 0; m < V; ++m) {
+    const int k = t + TT * m;
+    const int kappa = k <= N - 1 - N / 2 ? k : k - N;
+    const int i = kappa - k0r;
+    if (i >= 0 && i < wr) dst[(size_t)i * wc + j] = make_float2(v[m].x * scale, v[m].y * scale);
+  }
+}
+
+// -------------------------------------------------------------- coherent
+// Block (s, img, g): coarse rows t_r = s + P a (a < M = L/P), kernels g, g+G, ...
+// LDS: spectrum window of the image (one channel at a time), B[a][kc] (M x wc),
+// transform scratch.
+template <class PL, class PM>
+__global__ __launch_bounds__(kBlock) void k_fp_coherent(FpCoh A) {
+  constexpr int L = PL::N, M = PM::N, P = L / M;
+  constexpr int TL = PL::T, VL = PL::V, GL = groups_per_block<PL>();
+  constexpr int TM = PM::T, VM = PM::V, GM = groups_per_block<PM>();
+  constexpr int ROWS = (M + GL - 1) / GL;  // coarse rows per row group
+  extern __shared__ float2 lds[];
+  const int wr = A.wr, wc = A.wc;
+  float2* specL = lds;
+  float2* Bb = specL + wr * wc;
+  float2* scr = Bb + M * wc;
+  const int s = blockIdx.x, img = blockIdx.y, grp = blockIdx.z;
+  const int tid = threadIdx.x;
+
+  // row-stage geometry
+  const int gl = tid / TL, tl = tid - gl * TL;
+  const bool llive = gl < GL;
+  const int gll = llive ? gl : 0;
+  typename PL::Tw twl;
+  PL::load_tw(twl, tl, A.tw_l);
+  // column-stage geometry
+  const int gm = tid / TM, tm = tid - gm * TM;
+  const bool mlive = gm < GM;
+  const int gmm = mlive ? gm : 0;
+  typename PM::Tw twm;
+  if constexpr (!std::is_same<PL, PM>::value) PM::load_tw(twm, tm, A.tw_m);
+
+  float acc[ROWS][VL];
+#pragma unroll
+  for (int q = 0; q < ROWS; ++q)
+#pragma unroll
+    for (int m = 0; m < VL; ++m) acc[q][m] = 0.f;
+
+  for (int c = 0; c < A.nchan; ++c) {
+    const float2* sp = A.spec + ((size_t)img * A.nchan + c) * wr * wc;
+    __syncthreads();
+    for (int e = tid; e < wr * wc; e += kBlock) specL[e] = sp[e];
+    __syncthreads();
+    for (int kk = grp; kk < A.nkern; kk += A.groups) {
+      const float2* kp = A.kern + (size_t)kk * wr * wc;
+      // column stage: column kc handled by group gm (round-robin); M-point
+      // transform of the kr fold (kr mod M) twisted by exp(-2 pi i kr s / L)
+      for (int c0 = 0; c0 < wc; c0 += GM) {
+        const int col = c0 + gmm;
+        const bool on = mlive && col < wc;
+        float2 v[VM];
+#pragma unroll
+        for (int m = 0; m < VM; ++m) {
+          const int u = tm + TM * m;
+          float2 sum = make_float2(0.f, 0.f);
+          if (on) {
+            for (int ir = pmod(u - A.k0r, M); ir < wr; ir += M) {
+              const int kr = A.k0r + ir;
+              float2 a = reg::mul(kp[ir * wc + col], specL[ir * wc + col]);
+              if (P > 1) a = reg::mul(a, A.tw_l[pmod(kr * s, L)]);
+              sum.x += a.x;
+              sum.y += a.y;
+            }
+          }
+          v[m] = sum;
+        }
+        if constexpr (std::is_same<PL, PM>::value)
+          PM::template run<false>(v, scr + gm * PM::LDS, tm, twl);
+        else
+          PM::template run<false>(v, scr + gm * PM::LDS, tm, twm);
+        if (on) {
+#pragma unroll
+          for (int m = 0; m < VM; ++m) Bb[(tm + TM * m) * wc + col] = v[m];
+        }
+      }
+      __syncthreads();
+      // row stage: row a handled by group gl (a = gl + GL q); L-point transform
+      // of the kc fold (kc mod L); |.|^2 accumulated
+#pragma unroll
+      for (int q = 0; q < ROWS; ++q) {
+        const int a = gll + GL * q;
+        const bool on = llive && a < M;
+        float2 v[VL];
+#pragma unroll
+        for (int m = 0; m < VL; ++m) {
+          const int x = tl + TL * m;
+          float2 sum = make_float2(0.f, 0.f);
+          if (on)
+            for (int ic = pmod(x - A.k0c, L); ic < wc; ic += L) {
+              const float2 b = Bb[a * wc + ic];
+              sum.x += b.x;
+              sum.y += b.y;
+            }
+          v[m] = sum;
+        }
+        PL::template run<false>(v, scr + gl * PL::LDS, tl, twl);
+#pragma unroll
+        for (int m = 0; m < VL; ++m) acc[q][m] += v[m].x * v[m].x + v[m].y * v[m].y;
+      }
+      __syncthreads();
+    }
+  }
+  // epilogue: row DFT (+ sign) of the coarse rows, orders q in [0, qc] -> R
+  const int nq = A.qc + 1;
+#pragma unroll
+  for (int q = 0; q < ROWS; ++q) {
+    const int a = gll + GL * q;
+    const bool on = llive && a < M;
+    float2 v[VL];
+#pragma unroll
+    for (int m = 0; m < VL; ++m) v[m] = make_float2(acc[q][m], 0.f);
+    PL::template run<true>(v, scr + gl * PL::LDS, tl, twl);
+    if (on) {
+      float2* dst = A.R + (((size_t)img * A.groups + grp) * L + (s + P * a)) * nq;
+#pragma unroll
+      for (int m = 0; m < VL; ++m) {
+        const int x = tl + TL * m;
+        if (x < nq) dst[x] = v[m];
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------- band extraction
+// Ihat[img][q][i] = scale * sum_t (sum_g R[img][g][t][q]) e^{+2 pi i qr t / L},
+// qr = i - qr_max, i in [0, 2 qr_max + 1).
+template <class PL>
+__global__ __launch_bounds__(kBlock) void k_fp_band(const float2* __restrict__ R, int groups, int qc,
+                                                    int qrm, float scale,
+                                                    const float2* __restrict__ twt,
+                                                    float2* __restrict__ Ihat) {
+  constexpr int L = PL::N, TL = PL::T, VL = PL::V, G = groups_per_block<PL>();
+  extern __shared__ float2 lds[];
+  const int g = threadIdx.x / TL, t = threadIdx.x - g * TL;
+  const bool live = g < G;
+  const int gg = live ? g : 0;
+  const int img = blockIdx.y;
+  const int q = blockIdx.x * G + gg;
+  const int nq = qc + 1;
+  const bool on = live && q < nq;
+  typename PL::Tw tw;
+  PL::load_tw(tw, t, twt);
+  float2 v[VL];
+#pragma unroll
+  for (int m = 0; m < VL; ++m) {
+    float2 sum = make_float2(0.f, 0.f);
+    if (on)
+      for (int k = 0; k < groups; ++k) {
+        const float2 x = R[(((size_t)img * groups + k) * L + (t + TL * m)) * nq + q];
+        sum.x += x.x;
+        sum.y += x.y;
+      }
+    v[m] = sum;
+  }
+  PL::template run<true>(v, lds + g * PL::LDS, t, tw);
+  if (!on) return;
+  const int nb = 2 * qrm + 1;
+  float2* dst = Ihat + ((size_t)img * nq + q) * nb;
+#pragma unroll
+  for (int m = 0; m < VL; ++m) {
+    const int x = t + TL * m;
+    const int qr = x <= qrm ? x : x - L;
+    if (qr >= -qrm && qr <= qrm) dst[qr + qrm] = make_float2(v[m].x * scale, v[m].y * scale);
+  }
+}
+
+// U[img][q][y] = sum_qr Ihat[img][q][qr] e^{-2 pi i qr y / ny}
+template <class P>
+__global__ __launch_bounds__(kBlock) void k_fp_ucols(const float2* __restrict__ Ihat, int qc, int qrm,
+                                                     const float2* __restrict__ twt,
+                                                     float2* __restrict__ U) {
+  constexpr int N = P::N, TT = P::T, V = P::V, G = groups_per_block<P>();
+  extern __shared__ float2 lds[];
+  const int g = threadIdx.x / TT, t = threadIdx.x - g * TT;
+  const bool live = g < G;
+  const int gg = live ? g : 0;
+  const int img = blockIdx.y;
+  const int q = blockIdx.x * G + gg;
+  const int nq = qc + 1;
+  const bool on = live && q < nq;
+  typename P::Tw tw;
+  P::load_tw(tw, t, twt);
+  const float2* src = Ihat + ((size_t)img * nq + (on ? q : 0)) * (2 * qrm + 1);
+  float2 v[V];
+#pragma unroll
+  for (int m = 0; m < V; ++m) {
+    const int y = t + TT * m;
+    const int qr = y <= qrm ? y : y - N;
+    v[m] = (on && qr >= -qrm && qr <= qrm) ? src[qr + qrm] : make_float2(0.f, 0.f);
+  }
+  P::template run<false>(v, lds + g * P::LDS, t, tw);
+  if (!on) return;
+  float2* dst = U + ((size_t)img * nq + q) * N;
+#pragma unroll
+  for (int m = 0; m < V; ++m) dst[t + TT * m] = v[m];
+}
+
+// Image rows from U (Hermitian extension over q), two rows per transform.
+template <class P>
+__global__ __launch_bounds__(kBlock) void k_fp_urows(const float2* __restrict__ U, int ny, int qc,
+                                                     const float2* __restrict__ twt,
+                                                     float* __restrict__ out) {
+  constexpr int N = P::N, TT = P::T, V = P::V, G = groups_per_block<P>();
+  extern __shared__ float2 lds[];
+  const int nq = qc + 1;
+  float2* tile = lds;                         // [nq][2G]
+  float2* scr = lds + nq * 2 * G;
+  const int g = threadIdx.x / TT, t = threadIdx.x - g * TT;
+  const bool live = g < G;
+  const int gg = live ? g : 0;
+  const int img = blockIdx.y;
+  const int y0 = blockIdx.x * 2 * G;
+  for (int e = threadIdx.x; e < nq * 2 * G; e += kBlock) {
+    const int q = e / (2 * G), r = e - q * 2 * G;
+    const int y = y0 + r;
+    tile[e] = y < ny ? U[((size_t)img * nq + q) * ny + y] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  typename P::Tw tw;
+  P::load_tw(tw, t, twt);
+  float2 v[V];
+#pragma unroll
+  for (int m = 0; m < V; ++m) {
+    const int x = t + TT * m;
+    float2 z = make_float2(0.f, 0.f);
+    if (live && (x <= qc || x >= N - qc)) {
+      const bool neg = x > qc;
+      const int q = neg ? N - x : x;
+      float2 u0 = tile[q * 2 * G + 2 * gg], u1 = tile[q * 2 * G + 2 * gg + 1];
+      if (neg) {
+        u0.y = -u0.y;
+        u1.y = -u1.y;
+      }
+      z = make_float2(u0.x - u1.y, u0.y + u1.x);  // u0 + i u1
+    }
+    v[m] = z;
+  }
+  P::template run<false>(v, scr + g * P::LDS, t, tw);
+  if (!live) return;
+  const int ya = y0 + 2 * gg, yb = ya + 1;
+#pragma unroll
+  for (int m = 0; m < V; ++m) {
+    const int x = t + TT * m;
+    if (ya < ny) out[((size_t)img * ny + ya) * N + x] = v[m].x;
+    if (yb < ny) out[((size_t)img * ny + yb) * N + x] = v[m].y;
+  }
+}
+
+// ---------------------------------------------------------------- tables
+// Image sizes: N = R1 x R2.
+#define OI_IMAGE_PLANS(X) X(128, 8, 16) X(256, 16, 16) X(512, 16, 32) X(1024, 32, 32)
+// Coarse sizes: L = R1 x R2 (column stage uses the same plan when P = 1).
+#define OI_COARSE_PLANS(X)                                                                      \
+  X(16, 4, 4) X(25, 5, 5) X(32, 4, 8) X(36, 6, 6) X(48, 4, 12) X(64, 8, 8) X(75, 5, 15)         \
+  X(81, 9, 9) X(100, 10, 10) X(128, 8, 16) X(144, 12, 12)
+
+template <class P>
+size_t rows_lds() {
+  return (size_t)scratch_groups<P>() * P::LDS * sizeof(float2);
+}
+
+template <class K>
+static int allow(K kernel, size_t bytes) {
+  if (bytes > kLdsLimit) return fail(OI_ERR_UNSUPPORTED, "fast path: shared memory over 160 KiB");
+  if (bytes > 64 * 1024)
+    OI_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+  return OI_OK;
+}
+
+template <class P>
+static int run_rows(const FpArgs& a, hipStream_t st) {
+  constexpr int G = groups_per_block<P>();
+  const size_t lds = rows_lds<P>();
+  if (a.kind == OI_FIELD_REAL) {
+    int rc = allow(k_fp_rows_real<P>, lds);
+    if (rc) return rc;
+    dim3 grid((a.ny + 2 * G - 1) / (2 * G), a.batch);
+    hipLaunchKernelGGL(k_fp_rows_real<P>, grid, dim3(kBlock), lds, st,
+                       static_cast<const float*>(a.field), a.ny, a.k0c, a.wc, a.tw_x, a.T);
+  } else {
+    int rc = allow(k_fp_rows_c64<P>, lds);
+    if (rc) return rc;
+    dim3 grid((a.ny + G - 1) / G, a.batch * a.nchan);
+    hipLaunchKernelGGL(k_fp_rows_c64<P>, grid, dim3(kBlock), lds, st,
+                       static_cast<const float2*>(a.field), a.cin, a.chans, a.nchan, a.ny, a.k0c,
+                       a.wc, a.tw_x, a.T);
+  }
+  OI_LAUNCH_CHECK("fp_rows");
+  return OI_OK;
+}
+
+template <class P>
+static int run_cols(const FpArgs& a, hipStream_t st) {
+  constexpr int G = groups_per_block<P>();
+  const size_t lds = rows_lds<P>();
+  int rc = allow(k_fp_cols<P>, lds);
+  if (rc) return rc;
+  dim3 grid((a.wc + G - 1) / G, a.batch * a.nchan);
+  const float scale = 1.0f / ((float)a.ny * (float)a.nx);
+  hipLaunchKernelGGL(k_fp_cols<P>, grid, dim3(kBlock), lds, st, a.T, a.k0r, a.wr, a.wc, scale,
+                     a.tw_y, a.spec);
+  OI_LAUNCH_CHECK("fp_cols");
+  return OI_OK;
+}
+
+template <class P>
+static int run_ucols(const FpArgs& a, hipStream_t st) {
+  constexpr int G = groups_per_block<P>();
+  const size_t lds = rows_lds<P>();
+  int rc = allow(k_fp_ucols<P>, lds);
+  if (rc) return rc;
+  dim3 grid((a.qc + 1 + G - 1) / G, a.batch);
+  hipLaunchKernelGGL(k_fp_ucols<P>, grid, dim3(kBlock), lds, st, a.Ihat, a.qc, a.qr, a.tw_y, a.U);
+  OI_LAUNCH_CHECK("fp_ucols");
+  return OI_OK;
+}
+
+template <class P>
+static int run_urows(const FpArgs& a, hipStream_t st) {
+  constexpr int G = groups_per_block<P>();
+  const size_t lds = rows_lds<P>() + (size_t)(a.qc + 1) * 2 * G * sizeof(float2);
+  int rc = allow(k_fp_urows<P>, lds);
+  if (rc) return rc;
+  dim3 grid((a.ny + 2 * G - 1) / (2 * G), a.batch);
+  hipLaunchKernelGGL(k_fp_urows<P>, grid, dim3(kBlock), lds, st, a.U, a.ny, a.qc, a.tw_x, a.out);
+  OI_LAUNCH_CHECK("fp_urows");
+  return OI_OK;
+}
+
+template <class PL>
+static size_t coherent_lds(const FpArgs& a) {
+  using PM = PL;
+  const size_t scr = (size_t)scratch_groups<PL>() * PL::LDS;
+  const size_t scm = (size_t)scratch_groups<PM>() * PM::LDS;
+  return ((size_t)a.wr * a.wc + (size_t)PL::N * a.wc + (scr > scm ? scr : scm)) * sizeof(float2);
+}
+
+template <class PL>
+static int run_coherent(const FpArgs& a, const FpCoh& c, hipStream_t st) {
+  const size_t lds = coherent_lds<PL>(a);
+  int rc = allow(k_fp_coherent<PL, PL>, lds);
+  if (rc) return rc;
+  dim3 grid(1, a.batch, c.groups);
+  hipLaunchKernelGGL((k_fp_coherent<PL, PL>), grid, dim3(kBlock), lds, st, c);
+  OI_LAUNCH_CHECK("fp_coherent");
+  return OI_OK;
+}
+
+template <class PL>
+static int run_band(const FpArgs& a, int groups, hipStream_t st) {
+  constexpr int G = groups_per_block<PL>();
+  const size_t lds = rows_lds<PL>();
+  int rc = allow(k_fp_band<PL>, lds);
+  if (rc) return rc;
+  dim3 grid((a.qc + 1 + G - 1) / G, a.batch);
+  const float scale = 1.0f / ((float)PL::N * (float)PL::N);
+  hipLaunchKernelGGL(k_fp_band<PL>, grid, dim3(kBlock), lds, st, a.R, groups, a.qc, a.qr, scale,
+                     a.tw_l, a.Ihat);
+  OI_LAUNCH_CHECK("fp_band");
+  return OI_OK;
+}
+
+}  // namespace fp
+
+// ------------------------------------------------------------ dispatch
+bool fp_image_size_ok(int n) {
+#define X(NN, A, B) if (n == NN) return true;
+  OI_IMAGE_PLANS(X)
+#undef X
+  return false;
+}
+
+bool fp_coarse_size_ok(int l) {
+#define X(NN, A, B) if (l == NN) return true;
+  OI_COARSE_PLANS(X)
+#undef X
+  return false;
+}
+
+int fp_coarse_choose(int need) {
+  int best = 0;
+#define X(NN, A, B) if (NN >= need && (best == 0 || NN < best)) best = NN;
+  OI_COARSE_PLANS(X)
+#undef X
+  return best;
+}
+
+size_t fp_coherent_lds(int L, int wr, int wc) {
+  FpArgs a{};
+  a.wr = wr;
+  a.wc = wc;
+#define X(NN, A, B) if (L == NN) return fp::coherent_lds<reg::GFFT<NN, A, B>>(a);
+  OI_COARSE_PLANS(X)
+#undef X
+  return (size_t)-1;
+}
+
+int fp_coherent_rows_fit(int L) {
+  // accumulator registers per thread: ceil(L / groups) * V
+#define X(NN, A, B)                                                                      \
+  if (L == NN) {                                                                         \
+    using P = reg::GFFT<NN, A, B>;                                                       \
+    constexpr int G = fp::groups_per_block<P>();                                         \
+    return ((NN + G - 1) / G) * P::V <= 96;                                              \
+  }
+  OI_COARSE_PLANS(X)
+#undef X
+  return 0;
+}
+
+int fp_spectrum(const FpArgs& a, hipStream_t st) {
+  int rc = OI_ERR_UNSUPPORTED;
+#define X(NN, A, B) if (a.nx == NN) rc = fp::run_rows<reg::GFFT<NN, A, B>>(a, st);
+  OI_IMAGE_PLANS(X)
+#undef X
+  if (rc) return rc == OI_ERR_UNSUPPORTED ? fail(rc, "fast path: unsupported nx") : rc;
+  if (a.mark) a.mark(a.mark_ctx, 1);
+  rc = OI_ERR_UNSUPPORTED;
+#define X(NN, A, B) if (a.ny == NN) rc = fp::run_cols<reg::GFFT<NN, A, B>>(a, st);
+  OI_IMAGE_PLANS(X)
+#undef X
+  if (rc) return rc == OI_ERR_UNSUPPORTED ? fail(rc, "fast path: unsupported ny") : rc;
+  return OI_OK;
+}
+
+int fp_image(const FpArgs& a, const FpCoh& c, hipStream_t st) {
+  int rc = OI_ERR_UNSUPPORTED;
+#define X(NN, A, B) if (a.L == NN) rc = fp::run_coherent<reg::GFFT<NN, A, B>>(a, c, st);
+  OI_COARSE_PLANS(X)
+#undef X
+  if (rc) return rc == OI_ERR_UNSUPPORTED ? fail(rc, "fast path: unsupported coarse size") : rc;
+  if (a.mark) a.mark(a.mark_ctx, 3);
+  rc = OI_ERR_UNSUPPORTED;
+#define X(NN, A, B) if (a.L == NN) rc = fp::run_band<reg::GFFT<NN, A, B>>(a, c.groups, st);
+  OI_COARSE_PLANS(X)
+#undef X
+  if (rc) return rc;
+  rc = OI_ERR_UNSUPPORTED;
+#define X(NN, A, B) if (a.ny == NN) rc = fp::run_ucols<reg::GFFT<NN, A, B>>(a, st);
+  OI_IMAGE_PLANS(X)
+#undef X
+  if (rc) return rc;
+  if (a.mark) a.mark(a.mark_ctx, 4);
+  rc = OI_ERR_UNSUPPORTED;
+#define X(NN, A, B) if (a.nx == NN) rc = fp::run_urows<reg::GFFT<NN, A, B>>(a, st);
+  OI_IMAGE_PLANS(X)
+#undef X
+  if (rc) return rc;
+  if (a.mark) a.mark(a.mark_ctx, 5);
+  return OI_OK;
+}
+
+}  // namespace oi

@@ -1,0 +1,243 @@
+// Register-resident FFTs with compile-time sizes.
+//
+// DFT<N, INV>: an N-point DFT on N values held by one thread, fully unrolled
+// mixed-radix Cooley-Tukey (first factor 4, 2, 3 or 5).  Twiddle factors are
+// compile-time constants: exp(-+2 pi i m / N) is evaluated by constexpr series
+// in float64 and rounded once, so no table is read and nothing spills.
+//
+// GFFT<N, R1, R2>: an N = R1 R2 point transform shared by T = N / V threads
+// (V = max(R1, R2), the smaller radix divides the larger).  Thread t holds
+// x[t + T m], m in [0, V), on entry and X[t + T m] on exit (Stockham order).
+// Stage 1 (radix R1) runs on the thread's own values; its outputs pass once
+// through a padded LDS scratch of the group; stage 2 (radix R2) applies the
+// twiddles w_N^{j (i mod R1)} (held in registers, loaded once per thread) and
+// lands back in registers.  R2 = 1 means a single-thread transform (T = 1).
+#pragma once
+
+#include "oi_common.h"
+
+namespace oi {
+namespace reg {
+
+constexpr double kPi = 3.14159265358979323846264338327950288;
+
+constexpr double series_sin(double x) {
+  double x2 = x * x, term = x, s = x;
+  for (int k = 1; k < 16; ++k) {
+    term *= -x2 / ((2.0 * k) * (2.0 * k + 1.0));
+    s += term;
+  }
+  return s;
+}
+constexpr double series_cos(double x) {
+  double x2 = x * x, term = 1.0, s = 1.0;
+  for (int k = 1; k < 16; ++k) {
+    term *= -x2 / ((2.0 * k - 1.0) * (2.0 * k));
+    s += term;
+  }
+  return s;
+}
+// exp(-2 pi i m / n): argument reduced to [-pi, pi] in integer arithmetic.
+constexpr double root_re(long m, long n) {
+  m %= n;
+  if (m < 0) m += n;
+  return series_cos(2.0 * kPi * (double)(2 * m <= n ? m : m - n) / (double)n);
+}
+constexpr double root_im(long m, long n) {
+  m %= n;
+  if (m < 0) m += n;
+  return -series_sin(2.0 * kPi * (double)(2 * m <= n ? m : m - n) / (double)n);
+}
+
+template <int N>
+struct FirstFactor {
+  static constexpr int value = (N % 4 == 0 && N != 4) ? 4 : (N % 2 == 0 ? 2 : (N % 3 == 0 ? 3 : (N % 5 == 0 ? 5 : N)));
+};
+
+__device__ __forceinline__ float2 mul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+template <int N, bool INV>
+struct DFT;
+
+template <bool INV>
+struct DFT<1, INV> {
+  __device__ __forceinline__ static void run(float2*) {}
+};
+
+template <bool INV>
+struct DFT<2, INV> {
+  __device__ __forceinline__ static void run(float2* u) {
+    const float2 a = u[0], b = u[1];
+    u[0] = make_float2(a.x + b.x, a.y + b.y);
+    u[1] = make_float2(a.x - b.x, a.y - b.y);
+  }
+};
+
+template <bool INV>
+struct DFT<3, INV> {
+  __device__ __forceinline__ static void run(float2* u) {
+    const float h = 0.86602540378443864676f;
+    const float2 s = make_float2(u[1].x + u[2].x, u[1].y + u[2].y);
+    const float2 d = make_float2((u[1].x - u[2].x) * h, (u[1].y - u[2].y) * h);
+    const float2 t = make_float2(u[0].x - 0.5f * s.x, u[0].y - 0.5f * s.y);
+    const float2 r = INV ? make_float2(-d.y, d.x) : make_float2(d.y, -d.x);
+    u[0] = make_float2(u[0].x + s.x, u[0].y + s.y);
+    u[1] = make_float2(t.x + r.x, t.y + r.y);
+    u[2] = make_float2(t.x - r.x, t.y - r.y);
+  }
+};
+
+template <bool INV>
+struct DFT<4, INV> {
+  __device__ __forceinline__ static void run(float2* u) {
+    const float2 s02 = make_float2(u[0].x + u[2].x, u[0].y + u[2].y);
+    const float2 d02 = make_float2(u[0].x - u[2].x, u[0].y - u[2].y);
+    const float2 s13 = make_float2(u[1].x + u[3].x, u[1].y + u[3].y);
+    const float2 d13 = make_float2(u[1].x - u[3].x, u[1].y - u[3].y);
+    const float2 r = INV ? make_float2(-d13.y, d13.x) : make_float2(d13.y, -d13.x);
+    u[0] = make_float2(s02.x + s13.x, s02.y + s13.y);
+    u[2] = make_float2(s02.x - s13.x, s02.y - s13.y);
+    u[1] = make_float2(d02.x + r.x, d02.y + r.y);
+    u[3] = make_float2(d02.x - r.x, d02.y - r.y);
+  }
+};
+
+template <bool INV>
+struct DFT<5, INV> {
+  __device__ __forceinline__ static void run(float2* u) {
+    const float c1 = 0.30901699437494742410f, c2 = -0.80901699437494742410f;
+    const float s1 = 0.95105651629515357212f, s2 = 0.58778525229247312917f;
+    const float2 a1 = make_float2(u[1].x + u[4].x, u[1].y + u[4].y);
+    const float2 b1 = make_float2(u[1].x - u[4].x, u[1].y - u[4].y);
+    const float2 a2 = make_float2(u[2].x + u[3].x, u[2].y + u[3].y);
+    const float2 b2 = make_float2(u[2].x - u[3].x, u[2].y - u[3].y);
+    const float2 x0 = u[0];
+    const float2 t1 = make_float2(x0.x + c1 * a1.x + c2 * a2.x, x0.y + c1 * a1.y + c2 * a2.y);
+    const float2 t2 = make_float2(x0.x + c2 * a1.x + c1 * a2.x, x0.y + c2 * a1.y + c1 * a2.y);
+    const float2 v1 = make_float2(s1 * b1.x + s2 * b2.x, s1 * b1.y + s2 * b2.y);
+    const float2 v2 = make_float2(s2 * b1.x - s1 * b2.x, s2 * b1.y - s1 * b2.y);
+    const float2 r1 = INV ? make_float2(-v1.y, v1.x) : make_float2(v1.y, -v1.x);
+    const float2 r2 = INV ? make_float2(-v2.y, v2.x) : make_float2(v2.y, -v2.x);
+    u[0] = make_float2(x0.x + a1.x + a2.x, x0.y + a1.y + a2.y);
+    u[1] = make_float2(t1.x + r1.x, t1.y + r1.y);
+    u[4] = make_float2(t1.x - r1.x, t1.y - r1.y);
+    u[2] = make_float2(t2.x + r2.x, t2.y + r2.y);
+    u[3] = make_float2(t2.x - r2.x, t2.y - r2.y);
+  }
+};
+
+// N = F M, n = M f + m, k = k1 + F k2:
+//   Y[m][k1] = sum_f x[M f + m] w_F^{f k1};  Y[m][k1] *= w_N^{m k1};
+//   X[k1 + F k2] = sum_m Y[m][k1] w_M^{m k2}.
+template <int N, bool INV>
+struct DFT {
+  __device__ __forceinline__ static void run(float2* x) {
+    constexpr int F = FirstFactor<N>::value;
+    constexpr int M = N / F;
+    static_assert(F > 1 && F <= 5 && M * F == N, "DFT size must be a product of 2, 3 and 5");
+    float2 y[N];
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+      float2 t[F];
+#pragma unroll
+      for (int f = 0; f < F; ++f) t[f] = x[M * f + m];
+      DFT<F, INV>::run(t);
+#pragma unroll
+      for (int k1 = 0; k1 < F; ++k1) {
+        if (m * k1 == 0) {
+          y[m * F + k1] = t[k1];
+        } else {
+          const float wr = (float)root_re(m * k1, N);
+          const float wi = (float)(INV ? -root_im(m * k1, N) : root_im(m * k1, N));
+          y[m * F + k1] = mul(t[k1], make_float2(wr, wi));
+        }
+      }
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < F; ++k1) {
+      float2 t[M];
+#pragma unroll
+      for (int m = 0; m < M; ++m) t[m] = y[m * F + k1];
+      DFT<M, INV>::run(t);
+#pragma unroll
+      for (int k2 = 0; k2 < M; ++k2) x[k1 + F * k2] = t[k2];
+    }
+  }
+};
+
+// Padded LDS index: one spare slot every 32 entries breaks power-of-two strides.
+__device__ __forceinline__ int pad(int x) { return x + (x >> 5); }
+__host__ __device__ constexpr int padded(int n) { return n + (n >> 5) + 1; }
+
+template <int N_, int R1_, int R2_>
+struct GFFT {
+  static constexpr int N = N_;
+  static constexpr int R1 = R1_;
+  static constexpr int R2 = R2_;
+  static constexpr int V = R2 == 1 ? N : (R1 > R2 ? R1 : R2);
+  static constexpr int T = N / V;
+  static constexpr int U1 = V / R1;                 // stage-1 butterflies per thread
+  static constexpr int U2 = R2 == 1 ? 1 : V / R2;   // stage-2 butterflies per thread
+  static constexpr int LDS = R2 == 1 ? 0 : padded(N);  // scratch (float2) per group
+  static_assert(R2 == 1 ? R1 == N : (R1 * R2 == N && (V % R1) == 0 && (V % R2) == 0),
+                "GFFT: N = R1 R2 with the smaller radix dividing the larger");
+
+  // Stage-2 twiddles of thread t: w_N^{j (i mod R1)}, i = t + T u.
+  struct Tw {
+    float2 w[U2][R2 > 1 ? R2 - 1 : 1];
+  };
+
+  // table: exp(-2 pi i m / N), m in [0, N) (float64 values rounded once).
+  __device__ __forceinline__ static void load_tw(Tw& tw, int t, const float2* __restrict__ table) {
+    if constexpr (R2 > 1) {
+#pragma unroll
+      for (int u = 0; u < U2; ++u) {
+        const int k = (t + T * u) % R1;
+#pragma unroll
+        for (int j = 1; j < R2; ++j) tw.w[u][j - 1] = table[j * k];
+      }
+    }
+  }
+
+  // v: x[t + T m] in, X[t + T m] out.  lds: this group's scratch (LDS entries);
+  // every thread of the block must call (barriers inside) when R2 > 1.
+  template <bool INV>
+  __device__ __forceinline__ static void run(float2 (&v)[V], float2* lds, int t, const Tw& tw) {
+    if constexpr (R2 == 1) {
+      DFT<N, INV>::run(v);
+    } else {
+#pragma unroll
+      for (int u = 0; u < U1; ++u) {
+        float2 b[R1];
+#pragma unroll
+        for (int j = 0; j < R1; ++j) b[j] = v[u + j * U1];
+        DFT<R1, INV>::run(b);
+        const int i = t + T * u;
+#pragma unroll
+        for (int j = 0; j < R1; ++j) lds[pad(i * R1 + j)] = b[j];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int u = 0; u < U2; ++u) {
+        const int i = t + T * u;
+        float2 b[R2];
+#pragma unroll
+        for (int j = 0; j < R2; ++j) b[j] = lds[pad(i + j * (N / R2))];
+#pragma unroll
+        for (int j = 1; j < R2; ++j) {
+          const float2 w = tw.w[u][j - 1];
+          b[j] = mul(b[j], INV ? make_float2(w.x, -w.y) : w);
+        }
+        DFT<R2, INV>::run(b);
+#pragma unroll
+        for (int j = 0; j < R2; ++j) v[u + j * U2] = b[j];
+      }
+      __syncthreads();
+    }
+  }
+};
+
+}  // namespace reg
+}  // namespace oi

@@ -11,13 +11,9 @@
 //   k_coherent                  kernels x spectrum -> coarse |E|^2 rows,
 //                               followed by the row DFT of the coarse image
 //   k_interp_cols / _rows       coarse spectrum -> image (C2R, N points)
-#include "oi_internal.h"
+#include "oi_fastpath.h"
 
 namespace oi {
-
-struct Chans {
-  int c[4];
-};
 
 static size_t fft_lds(int nb, int n) { return 2ull * nb * n * sizeof(float2); }
 

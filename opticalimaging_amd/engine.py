@@ -34,6 +34,10 @@ class KernelSet:
     k0c: int
     ny: int
     nx: int
+    # (rows, cols) extent of one shifted pupil when the set holds every
+    # eigenpair of its TCC: the image band is then the TCC band, 2 floor(2 p),
+    # narrower than twice the kernels' support.  None otherwise.
+    band: tuple | None = None
 
     @property
     def K(self) -> int:
@@ -182,6 +186,13 @@ def field_encoding(field):
     return f.to(torch.complex64).contiguous(), nat.OI_FIELD_C64_CHLAST, cin, live
 
 
+def plan_for(ks: KernelSet, nchan: int = 1, max_batch: int = 1) -> Plan:
+    """Cached plan for imaging with kernel set `ks`."""
+    br, bc = ks.band if ks.band is not None else (0, 0)
+    return PLANS.get(ny=ks.ny, nx=ks.nx, k0r=ks.k0r, wr=ks.wr, k0c=ks.k0c, wc=ks.wc, nchan=nchan,
+                     max_batch=max_batch, band_r=min(br, ks.wr), band_c=min(bc, ks.wc))
+
+
 def socs_image(masks: torch.Tensor, ks: KernelSet, kind: int = nat.OI_FIELD_REAL, cin: int = 1,
                chans=(0,), out: torch.Tensor | None = None, max_batch: int | None = None,
                plan: Plan | None = None) -> torch.Tensor:
@@ -197,8 +208,7 @@ def socs_image(masks: torch.Tensor, ks: KernelSet, kind: int = nat.OI_FIELD_REAL
     chans = list(chans)
     nchan = len(chans)
     if plan is None:
-        plan = PLANS.get(ny=ny, nx=nx, k0r=ks.k0r, wr=ks.wr, k0c=ks.k0c, wc=ks.wc, nchan=nchan,
-                         max_batch=max(B, max_batch or 0))
+        plan = plan_for(ks, nchan, max(B, max_batch or 0))
     if out is None:
         out = torch.empty((B, ny, nx), dtype=torch.float32, device=masks.device)
     cptr, keep = nat.int_array(chans)

@@ -102,9 +102,11 @@ int oi_plan_destroy(oi_plan* plan);
 size_t oi_plan_bytes(const oi_plan* plan);
 
 /* Per-stage timing.  With slots > 0 every later imaging call on the plan
- * records HIP events on its stream at the boundaries of its five launches
+ * records HIP events on its stream at the boundaries of its five stages
  * (spectrum rows | spectrum columns | coherent sum | interpolation columns |
- * interpolation rows), keeping the last `slots` calls; slots = 0 turns it off.
+ * interpolation rows; on the compile-time-size path the interpolation-column
+ * stage is two launches, band extraction and column transform), keeping the
+ * last `slots` calls; slots = 0 turns it off.
  * oi_plan_read_timing waits for the recorded events and returns, per stage,
  * the summed milliseconds (stage_ms[5]) over *ncalls recorded calls.        */
 int oi_plan_set_timing(oi_plan* plan, int slots);

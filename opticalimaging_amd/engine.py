@@ -7,8 +7,8 @@ c64 array [K][wr][wc] holding each kernel at spectrum index kappa =
 restricted to the kernels' support, so imaging never touches the dense
 (ny, nx, K) array.
 
-``Imager`` owns one library plan per geometry and runs
-``oi_socs_image`` / ``oi_coherent_image`` on the current stream.
+``Plan`` owns one library plan per geometry (cached by ``plan_for``);
+``socs_image`` runs ``oi_socs_image`` on the current stream.
 """
 from __future__ import annotations
 

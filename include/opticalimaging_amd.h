@@ -243,6 +243,20 @@ int oi_zernike(const float* rho, const float* theta, int n, const double* coeffs
 int oi_char_matrix(const float* f, const float* g, int n, double na,
                    double index_image, void* out, void* stream);
 
+/* ------------------------------------------------------------- adjoint --
+ * Mask gradient of the SOCS image (no reference counterpart: the reference
+ * differentiates CalculateAerialImage_SOCS, CalculateAerialImage_SOCS.py:3-11,
+ * only through torch autograd).  For real masks (plan with nchan = 1):
+ *   grad_mask[b][y][x] = d/d mask[b][y][x]  sum_j grad_image[b][j] I[b][j],
+ * I = oi_socs_image(mask[b]) with the same kernels.  Coherent fields are
+ * recomputed per kernel on the plan's coarse grid; nothing is kept from the
+ * forward call.  mask, grad_image, grad_mask : float32 [batch][ny][nx];
+ * work : device scratch of oi_socs_adjoint_work_bytes(plan, batch, nkern)
+ * bytes (contents need no initialisation).                                  */
+size_t oi_socs_adjoint_work_bytes(const oi_plan* plan, int batch, int nkern);
+int oi_socs_adjoint(oi_plan* plan, const float* mask, const float* grad_image, int batch,
+                    const void* kern, int nkern, void* work, float* grad_mask, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,88 @@
+"""Differentiable SOCS imaging for mask optimisation (OPC / inverse lithography).
+
+``aerial_image(mask, kernels)`` is the SOCS intensity of
+``CalculateAerialImage_SOCS`` (``CalculateAerialImage_SOCS.py:3-11``) for a
+batch of real masks [B, Ny, Nx] (channel 0 of the reference's ScatterField),
+as a ``torch.autograd.Function``: the forward pass runs ``oi_socs_image`` and
+the backward pass the hand-written adjoint ``oi_socs_adjoint`` (conjugate-
+kernel correlation of the image cotangent with per-kernel fields recomputed on
+the coarse grid).  The reference has no gradient path of its own beyond torch
+autograd through its complex64 ops.
+
+``ilt`` runs plain gradient steps of the pixel-based ILT objective
+||I(sigmoid(theta)) - T||^2.
+"""
+from __future__ import annotations
+
+import torch
+
+from . import engine
+
+__all__ = ["SOCSImage", "aerial_image", "kernel_set", "ilt_loss", "ilt"]
+
+
+def kernel_set(kernels) -> engine.KernelSet:
+    """KernelSet from DecomposeTCC_SOCS output (dense (Ny, Nx, K)) or a KernelSet."""
+    if isinstance(kernels, engine.KernelSet):
+        return kernels
+    ks = getattr(kernels, "_oi_kernels", None)
+    if ks is not None:
+        return ks
+    return engine.KernelSet.from_dense(torch.as_tensor(kernels))
+
+
+class SOCSImage(torch.autograd.Function):
+    """I = SOCS image of real masks; backward = oi_socs_adjoint."""
+
+    @staticmethod
+    def forward(ctx, mask: torch.Tensor, ks: engine.KernelSet) -> torch.Tensor:
+        m = mask.detach().to(torch.float32).contiguous()
+        B = int(m.shape[0])
+        plan = engine.plan_for(ks, 1, B)
+        out = engine.socs_image(m, ks, plan=plan)
+        ctx.save_for_backward(m)
+        ctx.ks = ks
+        ctx.in_dtype = mask.dtype
+        return out
+
+    @staticmethod
+    def backward(ctx, grad_out: torch.Tensor):
+        (m,) = ctx.saved_tensors
+        g = engine.socs_adjoint(m, grad_out.detach().to(torch.float32).contiguous(), ctx.ks)
+        return g.to(ctx.in_dtype), None
+
+
+def aerial_image(mask: torch.Tensor, kernels) -> torch.Tensor:
+    """Differentiable SOCS intensity of masks [B, Ny, Nx] (or [Ny, Nx]) on the device."""
+    ks = kernel_set(kernels)
+    squeeze = mask.dim() == 2
+    m = mask.unsqueeze(0) if squeeze else mask
+    if m.dim() != 3:
+        raise ValueError("mask must be (Ny, Nx) or (B, Ny, Nx)")
+    out = SOCSImage.apply(m, ks)
+    return out[0] if squeeze else out
+
+
+def ilt_loss(theta: torch.Tensor, target: torch.Tensor, kernels) -> torch.Tensor:
+    """||I(sigmoid(theta)) - target||^2 summed over the batch."""
+    img = aerial_image(torch.sigmoid(theta), kernels)
+    return ((img - target) ** 2).sum()
+
+
+def ilt(theta: torch.Tensor, target: torch.Tensor, kernels, steps: int = 50, lr: float = 0.5):
+    """Normalised gradient descent on theta (each image's step is lr times its
+    gradient divided by the gradient's RMS); returns (theta, per-step losses
+    tensor on the device).  The loop stays on the device (no host
+    synchronisation per step)."""
+    ks = kernel_set(kernels)
+    theta = theta.detach().clone()
+    losses = torch.empty(steps, dtype=torch.float32, device=theta.device)
+    with torch.enable_grad():
+        for i in range(steps):
+            t = theta.requires_grad_(True)
+            loss = ilt_loss(t, target, ks)
+            (g,) = torch.autograd.grad(loss, t)
+            losses[i] = loss.detach()
+            rms = g.pow(2).mean(dim=(-2, -1), keepdim=True).sqrt() + 1e-30
+            theta = t.detach() - lr * g / rms
+    return theta, losses

@@ -88,6 +88,9 @@ SIGNATURES = {
                                   P_int, c_void_p]),
     "oi_zernike": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
     "oi_char_matrix": (c_int, [c_void_p, c_void_p, c_int, c_double, c_double, c_void_p, c_void_p]),
+    "oi_socs_adjoint_work_bytes": (c_size_t, [c_void_p, c_int, c_int]),
+    "oi_socs_adjoint": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_int, c_void_p,
+                                c_void_p, c_void_p]),
 }
 
 

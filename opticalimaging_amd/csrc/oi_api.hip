@@ -5,6 +5,7 @@
 #include <new>
 #include <vector>
 
+#include "oi_adjoint.h"
 #include "oi_fastpath.h"
 
 namespace oi {
@@ -306,12 +307,20 @@ static FpArgs fast_args(oi_plan* p, int batch, hipStream_t st) {
   return a;
 }
 
+static int spectrum_impl(oi_plan* p, const void* field, int kind, int cin, const int* chans,
+                         int batch, void* spec, hipStream_t st);
+
 extern "C" int oi_spectrum_window(oi_plan* p, const void* field, int kind, int cin,
                                   const int* chans, int batch, void* spec, void* stream) {
   int rc = check_field(p, field, kind, cin, chans, batch);
   if (rc) return rc;
   if (!spec) return fail(OI_ERR_INVALID, "null spec");
-  hipStream_t st = (hipStream_t)stream;
+  return spectrum_impl(p, field, kind, cin, chans, batch, spec, (hipStream_t)stream);
+}
+
+static int spectrum_impl(oi_plan* p, const void* field, int kind, int cin, const int* chans,
+                         int batch, void* spec, hipStream_t st) {
+  int rc;
   const oi_geometry& g = p->g;
   mark(p, 0, st);
   if (g.path) {
@@ -490,4 +499,50 @@ extern "C" int oi_spectrum_full(const void* field, int kind, int cin, const int*
   if (hipStreamSynchronize(st) != hipSuccess && !rc) rc = fail(OI_ERR_RUNTIME, "spectrum_full: sync failed");
   free_twiddles(pool);
   return rc;
+}
+
+// ------------------------------------------------------------- adjoint
+static AdjGeom adj_geom(const oi_plan* p) {
+  const oi_geometry& g = p->g;
+  AdjGeom a{};
+  a.ny = g.ny;
+  a.nx = g.nx;
+  a.k0r = g.k0r;
+  a.wr = g.wr;
+  a.k0c = g.k0c;
+  a.wc = g.wc;
+  a.lr = g.lr;
+  a.lc = g.lc;
+  a.qr = g.qr;
+  a.qc = g.qc;
+  a.direct = g.direct;
+  a.fx = p->fx;
+  a.fy = p->fy;
+  a.flr = p->flr;
+  a.flc = p->flc;
+  return a;
+}
+
+extern "C" size_t oi_socs_adjoint_work_bytes(const oi_plan* p, int batch, int nkern) {
+  if (!p || batch < 1 || nkern < 1) return 0;
+  return adjoint_work_bytes(adj_geom(p), batch, nkern);
+}
+
+extern "C" int oi_socs_adjoint(oi_plan* p, const float* mask, const float* grad_image, int batch,
+                               const void* kern, int nkern, void* work, float* grad_mask,
+                               void* stream) {
+  if (!p || !mask || !grad_image || !kern || !work || !grad_mask)
+    return fail(OI_ERR_INVALID, "socs_adjoint: null argument");
+  if (p->g.nchan != 1) return fail(OI_ERR_INVALID, "socs_adjoint: the plan must carry one channel (real masks)");
+  if (batch < 1 || batch > p->g.max_batch) return fail(OI_ERR_INVALID, "batch outside [1, max_batch]");
+  if (nkern < 1) return fail(OI_ERR_INVALID, "socs_adjoint: nkern must be >= 1");
+  hipStream_t st = (hipStream_t)stream;
+  const int chan0 = 0;
+  const int slots = p->slots;
+  p->slots = 0;  // stage timing covers the forward imaging calls only
+  int rc = spectrum_impl(p, mask, OI_FIELD_REAL, 1, &chan0, batch, p->spec, st);
+  p->slots = slots;
+  if (rc) return rc;
+  return launch_adjoint(adj_geom(p), grad_image, p->spec, batch, static_cast<const float2*>(kern),
+                        nkern, work, grad_mask, st);
 }

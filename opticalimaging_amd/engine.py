@@ -216,3 +216,32 @@ def socs_image(masks: torch.Tensor, ks: KernelSet, kind: int = nat.OI_FIELD_REAL
                                        nat.ptr(ks.win), ks.K, nat.ptr(out), nat.stream()))
     del keep
     return out
+
+
+def socs_adjoint(masks: torch.Tensor, grad_image: torch.Tensor, ks: KernelSet,
+                 out: torch.Tensor | None = None, plan: Plan | None = None) -> torch.Tensor:
+    """Mask gradient of the SOCS image: d/d masks of sum(grad_image * I(masks)).
+
+    masks, grad_image: float32 [B, ny, nx] on the device (real masks).  The
+    coherent fields are recomputed per kernel by the library (oi_socs_adjoint);
+    no forward state is reused."""
+    B = int(masks.shape[0])
+    ny, nx = int(masks.shape[1]), int(masks.shape[2])
+    if (ny, nx) != (ks.ny, ks.nx):
+        raise ValueError(f"mask grid {ny}x{nx} does not match kernel grid {ks.ny}x{ks.nx}")
+    if tuple(grad_image.shape) != (B, ny, nx):
+        raise ValueError("grad_image must have the shape of masks")
+    if masks.dtype != torch.float32 or grad_image.dtype != torch.float32:
+        raise ValueError("socs_adjoint takes float32 masks and image cotangents")
+    masks = masks.contiguous()
+    grad_image = grad_image.contiguous()
+    if plan is None:
+        plan = plan_for(ks, 1, B)
+    lib = nat.load()
+    nbytes = int(lib.oi_socs_adjoint_work_bytes(plan.handle, B, ks.K))
+    work = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=masks.device)
+    if out is None:
+        out = torch.empty((B, ny, nx), dtype=torch.float32, device=masks.device)
+    nat.check(lib.oi_socs_adjoint(plan.handle, nat.ptr(masks), nat.ptr(grad_image), B,
+                                  nat.ptr(ks.win), ks.K, nat.ptr(work), nat.ptr(out), nat.stream()))
+    return out

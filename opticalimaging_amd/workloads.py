@@ -166,3 +166,30 @@ def field_from_mask(mask):
     f = torch.zeros(m.shape + (3,), dtype=m.dtype)
     f[..., 0] = m
     return f
+
+
+# Differentiation workload: SOCS kernels of the vector quasar optics above at
+# focus 0, continuous masks sigmoid(theta), theta ~ N(0, 1), binary targets.
+ILT_STEP = 0.5          # step of normalised gradient descent in theta units
+
+
+def ilt_optics(order=24):
+    _, src, proj, num, _ = vector_abbe_512()
+    proj.FocusRange = torch.tensor([0])
+    num.ImageCalculationMethod = "hopkins"
+    num.Hopkins_SettingType = "order"
+    num.Hopkins_Order = order
+    num.TCC_Precision = "split_bf16"
+    return src, proj, num
+
+
+def ilt_problem(batch=16, n=1024, seed=5):
+    """(theta [batch, n, n] float32 ~ N(0, 1), targets [batch, n, n] float32 in {0, 1})
+    over an n x n grid of 4 nm pixels; targets are seeded Manhattan layouts."""
+    period = 4.0 * n
+    rng = np.random.default_rng(seed)
+    theta = rng.standard_normal((batch, n, n)).astype(np.float32)
+    T = np.empty((batch, n, n), dtype=np.float32)
+    for b in range(batch):
+        T[b] = manhattan(n, period, np.random.default_rng(seed * 7919 + b)).astype(np.float32)
+    return theta, T

@@ -361,3 +361,52 @@ def test_socs_clear_field_full_order():
     f[..., 0] = 1
     hop, _ = O.hopkins_image(src, scatter(f, 700.0, 700.0), proj, num)
     assert np.allclose(hop, O.clear_field_intensity(src, proj), atol=1e-12)
+
+
+# --------------------------------------------------------------- gradient pins
+def _random_kernels(ny, nx, K, w, seed):
+    """Dense (ny, nx, K) kernels supported on a w x w window around DC, in the
+    fftshifted embedding of DecomposeTCC_SOCS.py:27-31."""
+    rng = np.random.default_rng(seed)
+    k = np.zeros((ny, nx, K), dtype=np.complex128)
+    r0, c0 = ny // 2 - w // 2, nx // 2 - w // 2
+    k[r0:r0 + w, c0:c0 + w] = rng.standard_normal((w, w, K)) + 1j * rng.standard_normal((w, w, K))
+    return np.fft.fftshift(k, axes=(0, 1))
+
+
+def test_torch_socs_restatement_equals_numpy_oracle():
+    from oracle.socs_torch import socs_image_t
+    ny, nx = 24, 32
+    kern = _random_kernels(ny, nx, 3, 7, seed=4)
+    mask = np.random.default_rng(5).random((ny, nx))
+    f3 = np.zeros((ny, nx, 3))
+    f3[..., 0] = mask
+    ref = O.socs_image(f3, kern, 900.0, 700.0, 0.8, 193.0)
+    got = socs_image_t(torch.as_tensor(mask), torch.as_tensor(kern), 900.0, 700.0, 0.8, 193.0)
+    assert np.max(np.abs(got.numpy() - ref)) <= 1e-12 * np.max(np.abs(ref))
+
+
+def test_autograd_gradient_matches_central_differences():
+    """sum(g * I) is quadratic in the mask, so a central difference is exact up
+    to rounding: the float64 autograd gradient must match it to ~1e-9."""
+    from oracle.socs_torch import socs_mask_gradient
+    ny, nx = 16, 20
+    kern = _random_kernels(ny, nx, 4, 5, seed=6)
+    rng = np.random.default_rng(7)
+    mask = rng.random((ny, nx))
+    g = rng.standard_normal((ny, nx))
+    args = (1000.0, 800.0, 0.85, 193.0)
+    grad = socs_mask_gradient(mask, g, kern, *args)
+
+    def loss(m):
+        f3 = np.zeros((ny, nx, 3))
+        f3[..., 0] = m
+        return float(np.sum(g * O.socs_image(f3, kern, *args)))
+
+    eps = 1e-3
+    for (y, x) in [(0, 0), (3, 7), (15, 19), (8, 1)]:
+        mp, mm = mask.copy(), mask.copy()
+        mp[y, x] += eps
+        mm[y, x] -= eps
+        fd = (loss(mp) - loss(mm)) / (2 * eps)
+        assert abs(fd - grad[y, x]) <= 1e-9 * np.max(np.abs(grad)), (y, x, fd, grad[y, x])

@@ -20,7 +20,8 @@ import torch
 from . import _native as nat
 from . import optics
 
-__all__ = ["CalculateTCCMatrix", "GetTCCMatrix", "TCCFactor", "tcc_contract", "cartesian_to_polar"]
+__all__ = ["CalculateTCCMatrix", "GetTCCMatrix", "TCCFactor", "tcc_factor", "tcc_contract",
+           "cartesian_to_polar"]
 
 _PRECISIONS = {"split_bf16": nat.OI_TCC_SPLIT_BF16, "fp32": nat.OI_TCC_FP32, "fp64": nat.OI_TCC_FP64}
 
@@ -65,13 +66,9 @@ def tcc_contract(G: torch.Tensor, V: torch.Tensor, alpha: float, precision: int,
     return out
 
 
-def CalculateTCCMatrix(source, xypitch, projector, focus, numerics):
-    """Returns (TCC [M, M] complex64 on the device, [len(g), len(f)]).
-
-    Grid: f = k / p_0, g = k / p_1 with p_i = xypitch[i] NA / lambda,
-    k in [-N, N], N = ceil(2 p) (CalculateTCCMatrix.py:12-22) or
-    numerics.Hopkins_GridHalfWidth.  Row m = a len(g) + b holds (f[a], g[b])
-    (CalculateTCCMatrix.py:23); f pairs with the source X coordinate."""
+def tcc_factor(source, xypitch, projector, focus, numerics) -> TCCFactor:
+    """The low-rank factor of CalculateTCCMatrix's TCC (shifted pupils G,
+    source weights, scale) without the M x M contraction."""
     vector = optics.imaging_mode(numerics)
     lam = float(source.Wavelength)
     na = float(projector.NA)
@@ -86,9 +83,20 @@ def CalculateTCCMatrix(source, xypitch, projector, focus, numerics):
     ncomp = G.shape[0]
     G = G.reshape(ncomp, src.n, lenf * leng)
     alpha = float(projector.IndexImage) / src.weight
-    tcc = tcc_contract(G, src.V, alpha, precision_code(numerics))
-    tcc._oi_factor = TCCFactor(G=G, V=src.V, alpha=alpha, lenf=lenf, leng=leng, pitch=(p0, p1))
-    return tcc, [leng, lenf]
+    return TCCFactor(G=G, V=src.V, alpha=alpha, lenf=lenf, leng=leng, pitch=(p0, p1))
+
+
+def CalculateTCCMatrix(source, xypitch, projector, focus, numerics):
+    """Returns (TCC [M, M] complex64 on the device, [len(g), len(f)]).
+
+    Grid: f = k / p_0, g = k / p_1 with p_i = xypitch[i] NA / lambda,
+    k in [-N, N], N = ceil(2 p) (CalculateTCCMatrix.py:12-22) or
+    numerics.Hopkins_GridHalfWidth.  Row m = a len(g) + b holds (f[a], g[b])
+    (CalculateTCCMatrix.py:23); f pairs with the source X coordinate."""
+    factor = tcc_factor(source, xypitch, projector, focus, numerics)
+    tcc = tcc_contract(factor.G, factor.V, factor.alpha, precision_code(numerics))
+    tcc._oi_factor = factor
+    return tcc, [factor.leng, factor.lenf]
 
 
 def GetTCCMatrix(sourceData, shiftedPupil, numerics=None):

@@ -24,7 +24,7 @@ import torch
 from . import _native as nat
 from . import engine
 
-__all__ = ["DecomposeTCC_SOCS", "socs_count", "eigen_desc"]
+__all__ = ["DecomposeTCC_SOCS", "socs_count", "eigen_desc", "socs_kernel_set"]
 
 
 def socs_count(s: torch.Tensor, M: int, numerics) -> int:
@@ -74,42 +74,47 @@ def eigen_desc(H: torch.Tensor):
 def _window_kernels(tcc, M, numerics):
     """(K, M) c64 kernels u_k sqrt(s_k) in TCC-grid order, K, and the pupil
     band when the kernels hold every non-zero eigenpair (else None)."""
-    dev = tcc.device
-    lib = nat.load()
     factor = getattr(tcc, "_oi_factor", None)
     if factor is not None and factor.G.shape[-1] == M:
-        G = factor.G
-        ncomp, nsrc, _ = G.shape
-        n = ncomp * nsrc
-        w = (factor.V * factor.alpha).to(dev, torch.float64).contiguous()
-        H = torch.empty((n, n), dtype=torch.complex128, device=dev)
-        nat.check(lib.oi_tcc_gram(nat.ptr(G), nat.ptr(w), M, nsrc, ncomp, nat.ptr(H), nat.stream()))
-        s, Q = eigen_desc(H)
-        s_all = torch.zeros(M, dtype=torch.float64, device=dev)
-        r = min(n, M)
-        s_all[:r] = s[:r]
-        K = socs_count(s_all, M, numerics)
-        keff = min(K, r)
-        win = torch.zeros((max(keff, 1), M), dtype=torch.complex64, device=dev)
-        if keff > 0:
-            Qk = Q[:, :keff].contiguous()
-            nat.check(lib.oi_kernels_from_gram(nat.ptr(G), nat.ptr(w), M, nsrc, ncomp, nat.ptr(Qk),
-                                               keff, nat.ptr(win), nat.stream()))
-        band = None
-        if K >= n and factor.pitch[0] > 0:
-            # every eigenpair kept: sum_k phi_k phi_k^H is the TCC, whose image
-            # band is one pupil diameter, 2 floor(2 p) per axis
-            band = (int(math.floor(2 * factor.pitch[0])) + 1, int(math.floor(2 * factor.pitch[1])) + 1)
-        # Kernels past the TCC's rank are exactly zero and change no image; they
-        # are not materialised (with K = M - 1 on a large grid the reference's
-        # dense (Ny, Nx, K) array would not fit in memory).
-        K = max(keff, 1)
-        return win[:K], K, band
+        return _factor_kernels(factor, M, numerics)
     T = tcc.to(torch.complex128)
     s, U = eigen_desc(T)
     K = socs_count(s, M, numerics)
     win = (U[:, :K] * torch.sqrt(s[:K])[None, :]).transpose(0, 1).to(torch.complex64).contiguous()
     return win, K, None
+
+
+def _factor_kernels(factor, M, numerics):
+    """Kernels from the TCC factor through the n x n Gram matrix (float64)."""
+    lib = nat.load()
+    G = factor.G
+    dev = G.device
+    ncomp, nsrc, _ = G.shape
+    n = ncomp * nsrc
+    w = (factor.V * factor.alpha).to(dev, torch.float64).contiguous()
+    H = torch.empty((n, n), dtype=torch.complex128, device=dev)
+    nat.check(lib.oi_tcc_gram(nat.ptr(G), nat.ptr(w), M, nsrc, ncomp, nat.ptr(H), nat.stream()))
+    s, Q = eigen_desc(H)
+    s_all = torch.zeros(M, dtype=torch.float64, device=dev)
+    r = min(n, M)
+    s_all[:r] = s[:r]
+    K = socs_count(s_all, M, numerics)
+    keff = min(K, r)
+    win = torch.zeros((max(keff, 1), M), dtype=torch.complex64, device=dev)
+    if keff > 0:
+        Qk = Q[:, :keff].contiguous()
+        nat.check(lib.oi_kernels_from_gram(nat.ptr(G), nat.ptr(w), M, nsrc, ncomp, nat.ptr(Qk),
+                                           keff, nat.ptr(win), nat.stream()))
+    band = None
+    if K >= n and factor.pitch[0] > 0:
+        # every eigenpair kept: sum_k phi_k phi_k^H is the TCC, whose image
+        # band is one pupil diameter, 2 floor(2 p) per axis
+        band = (int(math.floor(2 * factor.pitch[0])) + 1, int(math.floor(2 * factor.pitch[1])) + 1)
+    # Kernels past the TCC's rank are exactly zero and change no image; they
+    # are not materialised (with K = M - 1 on a large grid the reference's
+    # dense (Ny, Nx, K) array would not fit in memory).
+    K = max(keff, 1)
+    return win[:K], K, band
 
 
 def DecomposeTCC_SOCS(TCCMatrix_Stacked, FG_ValidSize, Nfg, numerics):
@@ -141,6 +146,32 @@ def DecomposeTCC_SOCS(TCCMatrix_Stacked, FG_ValidSize, Nfg, numerics):
     ks.band = band
     dense._oi_kernels = ks
     return dense
+
+
+def socs_kernel_set(source, xypitch, projector, focus, numerics, Nfg) -> engine.KernelSet:
+    """Device-resident kernel set equal to
+    DecomposeTCC_SOCS(*CalculateTCCMatrix(source, xypitch, projector, focus,
+    numerics), Nfg, numerics)._oi_kernels, without forming the M x M TCC or
+    the dense (Ny, Nx, K) kernels (eigenpairs from the Gram matrix of the
+    TCC's factor).  For serving and optimisation loops on large grids."""
+    from .CalculateTCCMatrix import tcc_factor
+    factor = tcc_factor(source, xypitch, projector, focus, numerics)
+    nx, ny = int(Nfg[0]), int(Nfg[1])
+    lenf, leng = factor.lenf, factor.leng
+    if lenf > ny or leng > nx:
+        raise RuntimeError("TCC grid larger than the mask grid")
+    win, K, band = _factor_kernels(factor, lenf * leng, numerics)
+    win = win.reshape(K, lenf, leng)
+    r0 = (ny - lenf + 1) // 2
+    c0 = (nx - leng + 1) // 2
+    dense = None
+    if ny % 2 or nx % 2:
+        temp = torch.zeros((K, ny, nx), dtype=torch.complex64, device=win.device)
+        temp[:, r0:r0 + lenf, c0:c0 + leng] = win
+        dense = torch.fft.fftshift(temp, dim=(1, 2)).permute(1, 2, 0).contiguous()
+    ks = _kernel_set(win, r0, c0, ny, nx, dense)
+    ks.band = band
+    return ks
 
 
 def _kernel_set(win, r0, c0, ny, nx, dense):

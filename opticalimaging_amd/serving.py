@@ -1,0 +1,268 @@
+"""Continuous-batching aerial-image server over device-resident SOCS kernels.
+
+Requests are mask clips of several sizes (one size class per grid size, each
+with its own device-resident kernel set).  One host thread runs the loop:
+
+  * arrivals join a FIFO; a request is *admitted* when its device bytes (mask
+    in + image out, float32) fit in what the HBM budget leaves after the
+    resident kernel sets and plan scratch;
+  * admitted requests wait per size class; whenever one of S execution slots
+    (a HIP stream with its own plans) is free, the class holding the oldest
+    admitted request launches a batch of up to its max_batch requests --
+    requests that arrived while other batches were running join the next
+    launch, so work of different classes overlaps on the device;
+  * a batch's completion is detected with a HIP event query; its requests'
+    bytes are released and their latency (arrival -> event complete, observed
+    by the host) is recorded.
+
+A request's image never depends on the batch it was computed in: plans fix
+the kernel partitioning per size class, so alone and in the mix give the
+same bits.
+
+The device layer is an ``Executor``; ``GpuExecutor`` runs the library's
+imaging path on HIP streams, and tests of the scheduling logic use a
+host-only executor with simulated durations.
+"""
+from __future__ import annotations
+
+import collections
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+__all__ = ["Request", "Completed", "SizeClass", "Server", "GpuExecutor", "latency_stats",
+           "size_classes", "pinned_masks"]
+
+
+@dataclass
+class Request:
+    rid: int
+    mask: object            # (n, n) float32 host array (numpy or pinned torch)
+    arrival: float = 0.0    # seconds from the start of the run
+
+    @property
+    def n(self) -> int:
+        return int(self.mask.shape[0])
+
+
+@dataclass
+class Completed:
+    rid: int
+    n: int
+    image: object           # float32 (n, n) on the device
+    arrival: float
+    admitted: float
+    launched: float
+    done: float
+    batch: int
+
+    @property
+    def latency(self) -> float:
+        return self.done - self.arrival
+
+
+@dataclass
+class SizeClass:
+    n: int
+    kernels: object          # engine.KernelSet on the device
+    max_batch: int
+
+    @property
+    def request_bytes(self) -> int:
+        return 2 * 4 * self.n * self.n
+
+
+DEFAULT_MAX_BATCH = {128: 64, 256: 64, 512: 32, 1024: 16, 2048: 4}
+
+
+def size_classes(sizes, source, projector, numerics, pixel_nm=4.0, focus=0.0, max_batch=None):
+    """One SizeClass per clip size n: kernels of the optics for an n x n clip
+    of period n * pixel_nm (DecomposeTCC_SOCS.socs_kernel_set, resident on
+    the device)."""
+    from .DecomposeTCC_SOCS import socs_kernel_set
+    out = []
+    for n in sizes:
+        period = n * pixel_nm
+        ks = socs_kernel_set(source, [period, period], projector, focus, numerics, [n, n])
+        mb = (max_batch or {}).get(n, DEFAULT_MAX_BATCH.get(n, max(1, (1 << 22) // (n * n))))
+        out.append(SizeClass(n, ks, mb))
+    return out
+
+
+def pinned_masks(rids, sizes, make):
+    """Request masks generated straight into one pinned host allocation:
+    returns a list of (n, n) float32 pinned tensors (make(rid, n) -> array)."""
+    sizes = [int(n) for n in sizes]
+    total = sum(n * n for n in sizes)
+    pool = torch.empty(total, dtype=torch.float32, pin_memory=True)
+    out = []
+    off = 0
+    for rid, n in zip(rids, sizes):
+        t = pool[off:off + n * n].view(n, n)
+        t.numpy()[...] = make(rid, n)
+        out.append(t)
+        off += n * n
+    return out
+
+
+class GpuExecutor:
+    """Runs batches on `streams` HIP streams (default priority), one plan per
+    (size class, stream)."""
+
+    def __init__(self, classes, streams: int = 4):
+        from . import engine
+        self.engine = engine
+        self.streams = [torch.cuda.Stream() for _ in range(streams)]
+        self.plans = {}
+        self.resident = 0
+        for c in classes.values():
+            ks = c.kernels
+            self.resident += ks.win.numel() * 8
+            br, bc = ks.band if ks.band is not None else (0, 0)
+            for si in range(streams):
+                p = engine.Plan(ks.ny, ks.nx, ks.k0r, ks.wr, ks.k0c, ks.wc, nchan=1,
+                                max_batch=c.max_batch, band_r=min(br, ks.wr),
+                                band_c=min(bc, ks.wc))
+                self.plans[(c.n, si)] = p
+                self.resident += p.bytes
+
+    @property
+    def slots(self) -> int:
+        return len(self.streams)
+
+    def launch(self, cls: SizeClass, slot: int, reqs):
+        s = self.streams[slot]
+        b = len(reqs)
+        n = cls.n
+        srcs = [torch.as_tensor(r.mask) for r in reqs]
+        with torch.cuda.stream(s):
+            dev_in = torch.empty((b, n, n), dtype=torch.float32, device="cuda")
+            for i, m in enumerate(srcs):      # pinned masks: asynchronous copies on s
+                dev_in[i].copy_(m, non_blocking=m.is_pinned())
+            out = self.engine.socs_image(dev_in, cls.kernels, out=torch.empty_like(dev_in),
+                                         plan=self.plans[(n, slot)])
+            ev = torch.cuda.Event()
+            ev.record(s)
+        return {"event": ev, "out": out, "keep": (srcs, dev_in)}
+
+    def done(self, handle) -> bool:
+        return handle["event"].query()
+
+    def outputs(self, handle):
+        out = handle["out"]
+        return [out[i] for i in range(out.shape[0])]
+
+
+@dataclass
+class _Batch:
+    cls: SizeClass
+    slot: int
+    reqs: list
+    launched: float
+    handle: object
+
+
+@dataclass
+class _Entry:
+    req: Request
+    admitted: float = 0.0
+
+
+class Server:
+    """Continuous batching under an HBM budget (bytes, including the resident
+    kernel sets and plan scratch reported by the executor)."""
+
+    def __init__(self, classes, executor, hbm_budget: float = 8e9):
+        self.classes = {c.n: c for c in classes} if not isinstance(classes, dict) else classes
+        self.ex = executor
+        self.budget = float(hbm_budget)
+        self.resident = float(getattr(executor, "resident", 0))
+        if self.resident >= self.budget:
+            raise ValueError(f"resident kernels and plans ({self.resident:.3e} B) exceed the HBM budget")
+        for c in self.classes.values():
+            if c.request_bytes > self.budget - self.resident:
+                raise ValueError(f"a {c.n}^2 request does not fit in the HBM budget")
+        self.in_use = 0.0
+        self.peak_in_use = 0.0
+        self.waiting = collections.deque()
+        self.pending = {n: collections.deque() for n in self.classes}
+        self.running = []
+        self.free_slots = list(range(executor.slots))
+        self.completed = []
+
+    # -- one pass of the loop -------------------------------------------------
+    def _complete(self, now):
+        still = []
+        for b in self.running:
+            if self.ex.done(b.handle):
+                outs = self.ex.outputs(b.handle)
+                for e, img in zip(b.reqs, outs):
+                    self.completed.append(Completed(e.req.rid, e.req.n, img, e.req.arrival,
+                                                    e.admitted, b.launched, now, len(b.reqs)))
+                    self.in_use -= b.cls.request_bytes
+                self.free_slots.append(b.slot)
+            else:
+                still.append(b)
+        self.running = still
+
+    def _admit(self, now):
+        while self.waiting:
+            r = self.waiting[0]
+            c = self.classes[r.n]
+            if self.resident + self.in_use + c.request_bytes > self.budget:
+                break
+            self.waiting.popleft()
+            self.in_use += c.request_bytes
+            self.peak_in_use = max(self.peak_in_use, self.in_use)
+            self.pending[r.n].append(_Entry(r, now))
+
+    def _launch(self, now):
+        while self.free_slots:
+            heads = [(q[0].req.arrival, n) for n, q in self.pending.items() if q]
+            if not heads:
+                return
+            _, n = min(heads)
+            c = self.classes[n]
+            q = self.pending[n]
+            take = [q.popleft() for _ in range(min(c.max_batch, len(q)))]
+            slot = self.free_slots.pop(0)
+            handle = self.ex.launch(c, slot, [e.req for e in take])
+            self.running.append(_Batch(c, slot, take, now, handle))
+
+    def step(self, now):
+        self._complete(now)
+        self._admit(now)
+        self._launch(now)
+
+    # -- driving a request stream ---------------------------------------------
+    def run(self, requests, clock=None, idle_sleep: float = 2e-5):
+        """Serve `requests` (sorted by arrival, seconds from the start) in real
+        time; returns the Completed records in completion order."""
+        for r in requests:
+            if self.classes.get(r.n) is None:
+                raise ValueError(f"no size class for {r.n}^2 requests")
+        clock = clock or time.perf_counter
+        t0 = clock()
+        i = 0
+        total = len(requests)
+        while len(self.completed) < total:
+            now = clock() - t0
+            while i < total and requests[i].arrival <= now:
+                self.waiting.append(requests[i])
+                i += 1
+            before = (len(self.completed), len(self.running))
+            self.step(now)
+            if (len(self.completed), len(self.running)) == before and idle_sleep:
+                time.sleep(idle_sleep)
+        return self.completed
+
+
+def latency_stats(done):
+    lat = np.array([c.latency for c in done]) * 1e3
+    span = max(c.done for c in done) - min(c.arrival for c in done)
+    return {"requests": len(done), "p50_ms": float(np.percentile(lat, 50)),
+            "p99_ms": float(np.percentile(lat, 99)), "mean_ms": float(lat.mean()),
+            "max_ms": float(lat.max()), "images_per_s": len(done) / span if span > 0 else None,
+            "mean_batch": float(np.mean([c.batch for c in done]))}

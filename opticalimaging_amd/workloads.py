@@ -65,24 +65,29 @@ def manhattan(n, period, rng, cd_range=(45.0, 90.0), lines=None):
     horizontal lines plus shorter 2D segments, widths ~ U(cd_range) nm."""
     pitch = period / n
     m = np.zeros((n, n), dtype=bool)
+    cols = np.zeros(n, dtype=bool)      # union of vertical 1D lines
+    rows = np.zeros(n, dtype=bool)      # union of horizontal 1D lines
     count = lines if lines is not None else max(4, int(period / 200))
     for i in range(count):
         cd = rng.uniform(*cd_range)
         start = rng.uniform(0, period)
         kind = rng.integers(0, 3)
         if kind == 0:       # vertical 1D line
-            m |= _wrap_range(n, pitch, start, cd, period)[None, :]
+            cols |= _wrap_range(n, pitch, start, cd, period)
         elif kind == 1:     # horizontal 1D line
-            m |= _wrap_range(n, pitch, start, cd, period)[:, None]
+            rows |= _wrap_range(n, pitch, start, cd, period)
         else:               # 2D segment
             length = rng.uniform(2 * cd, max(4 * cd, period / 2))
             other = rng.uniform(0, period)
             if rng.integers(0, 2):
-                m |= (_wrap_range(n, pitch, other, length, period)[:, None]
-                      & _wrap_range(n, pitch, start, cd, period)[None, :])
+                r = _wrap_range(n, pitch, other, length, period)
+                c = _wrap_range(n, pitch, start, cd, period)
             else:
-                m |= (_wrap_range(n, pitch, start, cd, period)[:, None]
-                      & _wrap_range(n, pitch, other, length, period)[None, :])
+                r = _wrap_range(n, pitch, start, cd, period)
+                c = _wrap_range(n, pitch, other, length, period)
+            m[np.ix_(np.nonzero(r)[0], np.nonzero(c)[0])] = True
+    m |= cols[None, :]
+    m |= rows[:, None]
     return m
 
 
@@ -193,3 +198,22 @@ def ilt_problem(batch=16, n=1024, seed=5):
     for b in range(batch):
         T[b] = manhattan(n, period, np.random.default_rng(seed * 7919 + b)).astype(np.float32)
     return theta, T
+
+
+# Serving workload: Poisson arrivals, clip sizes uniform over SERVING_SIZES,
+# masks as the attenuated-PSM layouts above (4 nm pixels) from seed = request id.
+SERVING_SIZES = (256, 512, 1024, 2048)
+
+
+def serving_arrivals(count, rate, sizes=SERVING_SIZES, seed=17):
+    """(arrival seconds, clip size) per request: exponential inter-arrival
+    times at `rate` requests/s and sizes drawn uniformly, both seeded."""
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0 / rate, size=count)
+    t = np.cumsum(gaps) - gaps[0]
+    n = rng.choice(np.asarray(sizes), size=count)
+    return t, n
+
+
+def serving_mask(rid, n):
+    return att_psm_mask(n, 4.0 * n, seed=int(rid))

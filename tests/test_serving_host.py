@@ -1,0 +1,142 @@
+"""Scheduling logic of the continuous-batching server, with a host-only
+executor whose batches take a simulated time (no GPU)."""
+import numpy as np
+import pytest
+
+from opticalimaging_amd.serving import Request, Server, SizeClass, latency_stats
+
+
+class SimExecutor:
+    """Batches finish `base + per_image * b` simulated seconds after launch."""
+
+    def __init__(self, clock, slots=2, base=1e-3, per_image=2e-4, resident=0):
+        self.clock = clock
+        self.slots = slots
+        self.base = base
+        self.per_image = per_image
+        self.resident = resident
+        self.launches = []
+
+    def launch(self, cls, slot, reqs):
+        now = self.clock[0]
+        scale = (cls.n / 256) ** 2
+        end = now + self.base + self.per_image * scale * len(reqs)
+        self.launches.append((now, end, cls.n, slot, [r.rid for r in reqs]))
+        return {"end": end, "imgs": [("img", r.rid) for r in reqs]}
+
+    def done(self, h):
+        return self.clock[0] >= h["end"]
+
+    def outputs(self, h):
+        return h["imgs"]
+
+
+def _classes(max_batch=8):
+    return [SizeClass(n, None, max_batch) for n in (256, 512, 1024)]
+
+
+def _requests(count, rate, seed=0):
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1 / rate, count))
+    sizes = rng.choice([256, 512, 1024], count)
+    return [Request(i, np.zeros((n, n), np.float32), float(a)) for i, (a, n) in enumerate(zip(t, sizes))]
+
+
+def _drive(server, ex, reqs, dt=5e-5, limit=1e6):
+    clock = ex.clock
+    i = 0
+    steps = 0
+    inflight_max = 0
+    while len(server.completed) < len(reqs):
+        while i < len(reqs) and reqs[i].arrival <= clock[0]:
+            server.waiting.append(reqs[i])
+            i += 1
+        server.step(clock[0])
+        inflight = sum(len(b.reqs) for b in server.running)
+        inflight_max = max(inflight_max, inflight)
+        assert server.resident + server.in_use <= server.budget + 1e-9
+        clock[0] += dt
+        steps += 1
+        assert steps < limit, "server made no progress"
+    return inflight_max
+
+
+def test_all_requests_complete_with_ordered_timestamps():
+    clock = [0.0]
+    ex = SimExecutor(clock)
+    srv = Server(_classes(), ex, hbm_budget=1e9)
+    reqs = _requests(300, rate=2000)
+    _drive(srv, ex, reqs)
+    done = srv.completed
+    assert sorted(c.rid for c in done) == list(range(300))
+    for c in done:
+        assert c.arrival <= c.admitted <= c.launched <= c.done
+    st = latency_stats(done)
+    assert st["requests"] == 300 and st["p50_ms"] <= st["p99_ms"]
+
+
+def test_batches_are_single_class_and_bounded():
+    clock = [0.0]
+    ex = SimExecutor(clock, slots=3)
+    srv = Server(_classes(max_batch=5), ex, hbm_budget=1e9)
+    reqs = _requests(200, rate=5000, seed=1)
+    _drive(srv, ex, reqs)
+    by_rid = {r.rid: r.n for r in reqs}
+    for _, _, n, _, rids in ex.launches:
+        assert 1 <= len(rids) <= 5
+        assert all(by_rid[r] == n for r in rids)
+    # a slot is never used by two batches at once
+    for slot in range(3):
+        spans = sorted((a, b) for a, b, _, s, _ in ex.launches if s == slot)
+        for (a0, b0), (a1, b1) in zip(spans, spans[1:]):
+            assert a1 >= b0
+
+
+def test_hbm_budget_limits_admission():
+    clock = [0.0]
+    ex = SimExecutor(clock, slots=4)
+    cls = [SizeClass(1024, None, 16)]
+    per = cls[0].request_bytes
+    srv = Server(cls, ex, hbm_budget=3 * per + 10)      # three requests fit
+    reqs = [Request(i, np.zeros((1024, 1024), np.float32), 0.0) for i in range(20)]
+    inflight_max = _drive(srv, ex, reqs)
+    assert inflight_max <= 3
+    assert srv.peak_in_use <= 3 * per
+    assert len(srv.completed) == 20
+
+
+def test_continuous_batching_overlaps_classes():
+    """While a long batch of one class runs, requests of another class that
+    arrive later are launched on a free slot before it finishes."""
+    clock = [0.0]
+    ex = SimExecutor(clock, slots=2, base=0.0, per_image=1e-3)
+    srv = Server(_classes(max_batch=4), ex, hbm_budget=1e9)
+    reqs = [Request(i, np.zeros((1024, 1024), np.float32), 0.0) for i in range(4)]
+    reqs += [Request(4 + i, np.zeros((256, 256), np.float32), 1e-3 + 1e-4 * i) for i in range(3)]
+    _drive(srv, ex, reqs, dt=1e-5)
+    big = [l for l in ex.launches if l[2] == 1024][0]
+    small = [l for l in ex.launches if l[2] == 256]
+    assert small[0][0] < big[1], "small clips waited for the running batch"
+    done = {c.rid: c for c in srv.completed}
+    assert done[4].done < done[0].done
+
+
+def test_oldest_class_first():
+    clock = [0.0]
+    ex = SimExecutor(clock, slots=1, base=1e-3, per_image=0.0)
+    srv = Server(_classes(max_batch=2), ex, hbm_budget=1e9)
+    reqs = [Request(0, np.zeros((512, 512), np.float32), 0.0),
+            Request(1, np.zeros((256, 256), np.float32), 1e-4),
+            Request(2, np.zeros((512, 512), np.float32), 2e-4),
+            Request(3, np.zeros((256, 256), np.float32), 3e-4)]
+    _drive(srv, ex, reqs, dt=1e-5)
+    order = [l[2] for l in ex.launches]
+    assert order[0] == 512 and order[1] == 256
+
+
+def test_budget_too_small_raises():
+    clock = [0.0]
+    with pytest.raises(ValueError):
+        Server([SizeClass(2048, None, 4)], SimExecutor(clock), hbm_budget=1e6)
+    with pytest.raises(ValueError):
+        Server([SizeClass(256, None, 4)], SimExecutor(clock, resident=2e9), hbm_budget=1e9)

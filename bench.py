@@ -17,6 +17,9 @@ Also reported, outside the timed region:
   * the plain GPU Abbe baseline (per-source torch.fft restatement of the
     reference's CalculateAbbeImage) on the same masks, and the speed-up;
   * the TCC contraction time and MFMA utilisation;
+  * serving: a Poisson stream of 256^2..2048^2 clips, continuous batching
+    under an 8 GB HBM budget, p50/p99 latency and images/s;
+  * ILT: 50 gradient steps of ||I(sigmoid(theta)) - T||^2 at 1024^2 x 16;
   * the float64 CPU oracle timed on a bounded sample (cpu_baseline).
 """
 from __future__ import annotations
@@ -54,6 +57,11 @@ def parse():
                     help="masks in the CPU K=24 SOCS sample (a third of them for CPU Abbe)")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip accuracy, plain-Abbe, TCC and CPU legs (profiling runs)")
+    ap.add_argument("--serving-requests", type=int, default=1200,
+                    help="requests in the serving leg (0 skips it)")
+    ap.add_argument("--serving-rate", type=float, default=3000.0,
+                    help="offered Poisson rate of the serving leg (requests/s)")
+    ap.add_argument("--ilt-steps", type=int, default=50, help="gradient steps of the ILT leg (0 skips)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 device copies of MIB MiB (calibrates HBM byte counters)")
     return ap.parse_args()
@@ -167,6 +175,77 @@ def rel_err(a, b):
     a = np.asarray(a, dtype=np.float64)
     b = np.asarray(b, dtype=np.float64)
     return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+
+def serving_leg(count, rate):
+    """Poisson stream of clips of 256^2..2048^2 (4 nm pixels, attenuated-PSM
+    layouts, seed = request id) against device-resident K=24 kernel sets of the
+    vector quasar optics, continuous batching under an 8 GB HBM budget."""
+    from opticalimaging_amd import serving, workloads
+    src, proj, num = workloads.ilt_optics()
+    t = time.perf_counter()
+    classes = serving.size_classes(workloads.SERVING_SIZES, src, proj, num)
+    torch.cuda.synchronize()
+    t_ks = time.perf_counter() - t
+    arr, sizes = workloads.serving_arrivals(count, rate)
+    t = time.perf_counter()
+    masks = serving.pinned_masks(range(count), sizes, workloads.serving_mask)
+    t_gen = time.perf_counter() - t
+    ex = serving.GpuExecutor({c.n: c for c in classes}, streams=4)
+    budget = 8e9
+    # warm-up: one batch per class
+    serving.Server(classes, ex, hbm_budget=budget).run(
+        [serving.Request(i, masks[i], 0.0) for i in range(min(count, 16))])
+    torch.cuda.synchronize()
+    # capacity: the first 256 clips all present at t = 0
+    nb = min(count, 256)
+    srv = serving.Server(classes, ex, hbm_budget=budget)
+    done = srv.run([serving.Request(i, masks[i], 0.0) for i in range(nb)])
+    cap = serving.latency_stats(done)
+    # offered load
+    srv = serving.Server(classes, ex, hbm_budget=budget)
+    done = srv.run([serving.Request(i, masks[i], float(arr[i])) for i in range(count)])
+    st = serving.latency_stats(done)
+    per = {}
+    for n in workloads.SERVING_SIZES:
+        lat = np.array([c.latency for c in done if c.n == n]) * 1e3
+        if lat.size:
+            per[str(n)] = {"requests": int(lat.size), "p50_ms": round(float(np.percentile(lat, 50)), 3),
+                           "p99_ms": round(float(np.percentile(lat, 99)), 3)}
+    return {"offered_rate_per_s": rate, "requests": count, "sizes": list(workloads.SERVING_SIZES),
+            "kernels": {str(c.n): {"K": c.kernels.K, "window": c.kernels.wr, "max_batch": c.max_batch}
+                        for c in classes},
+            "hbm_budget_bytes": budget, "resident_bytes": int(ex.resident),
+            "peak_in_flight_bytes": int(srv.peak_in_use),
+            "images_per_s": round(st["images_per_s"], 1), "p50_ms": round(st["p50_ms"], 3),
+            "p99_ms": round(st["p99_ms"], 3), "mean_batch": round(st["mean_batch"], 2),
+            "per_size": per,
+            "capacity_images_per_s": round(cap["images_per_s"], 1),
+            "kernel_build_s": round(t_ks, 2), "mask_generation_s": round(t_gen, 2)}
+
+
+def ilt_leg(steps):
+    """||I(sigmoid(theta)) - T||^2 descent at 1024^2 x 16 with the adjoint kernel."""
+    from opticalimaging_amd import ILT, workloads
+    from opticalimaging_amd.DecomposeTCC_SOCS import socs_kernel_set
+    src, proj, num = workloads.ilt_optics()
+    ks = socs_kernel_set(src, [4096.0, 4096.0], proj, 0.0, num, [1024, 1024])
+    theta_np, T_np = workloads.ilt_problem(batch=16, n=1024)
+    theta = torch.as_tensor(theta_np).cuda()
+    T = torch.as_tensor(T_np).cuda()
+    ILT.ilt(theta, T, ks, steps=2)            # warm-up
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    _, losses = ILT.ilt(theta, T, ks, steps=steps)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t
+    lo = losses.cpu().numpy()
+    b_alg = 16 * 1024 * 1024 * 12            # theta + target in, gradient out (float32)
+    return {"steps": steps, "batch": 16, "image": [1024, 1024], "kernels": ks.K,
+            "ms_per_step": round(dt / steps * 1e3, 3),
+            "images_per_s": round(16 * steps / dt, 1),
+            "hbm_alg_GBps": round(b_alg / (dt / steps) / 1e9, 1),
+            "loss_first": float(lo[0]), "loss_last": float(lo[-1])}
 
 
 def main():
@@ -330,6 +409,16 @@ def main():
         gpu_img0 = out[0].double().cpu().numpy()
         gpu24_img0 = out24[0].double().cpu().numpy()
         del tcc, kern_full, kern24
+
+        # ---------------- serving: mixed-size clips, continuous batching
+        if args.serving_requests > 0:
+            line["serving"] = serving_leg(args.serving_requests, args.serving_rate)
+            log(f"serving: {json.dumps(line['serving'])}")
+
+        # ---------------- ILT: 50 gradient steps at 1024^2 x 16
+        if args.ilt_steps > 0:
+            line["ilt"] = ilt_leg(args.ilt_steps)
+            log(f"ilt: {json.dumps(line['ilt'])}")
 
         # ---------------- CPU baseline: the float64 oracle, timed on a bounded
         # sample; its outputs also serve as the checker for the accuracy fields.

@@ -256,25 +256,107 @@ __global__ __launch_bounds__(kBlock) void k_fp_coherent(FpCoh A) {
     for (int m = 0; m < VL; ++m) v[m] = make_float2(acc[q][m], 0.f);
     PL::template run<true>(v, scr + gl * PL::LDS, tl, twl);
     if (on) {
-      float2* dst = A.R + (((size_t)img * A.groups + grp) * L + (s + P * a)) * nq;
+      float2* dst = A.R + ((size_t)img * A.groups + grp) * nq * L + (s + P * a);
 #pragma unroll
       for (int m = 0; m < VL; ++m) {
         const int x = tl + TL * m;
-        if (x < nq) dst[x] = v[m];
+        if (x < nq) dst[(size_t)x * L] = v[m];
       }
     }
   }
 }
 
+// Coherent sum, one workgroup per (image, kernel group), every coarse row and
+// every window column in one pass: NT = (groups of T_L threads) covers L rows.
+// All transforms exchange in place through one LDS tile [L][S] (S = padded L):
+// the column stage of window column kc works in tile column (k0c + kc) mod L,
+// the row stage of coarse row a in tile row a.  Requires wr, wc <= L (the
+// window folds onto the coarse grid without collisions).
+template <class PL, int NT>
+__global__ __launch_bounds__(NT) void k_fp_coh(FpCoh A) {
+  constexpr int L = PL::N, TL = PL::T, VL = PL::V;
+  constexpr int S = reg::padded(L);
+  static_assert(L * TL <= NT, "one group per coarse row");
+  static_assert(VL <= 32, "row mask holds VL bits");
+  extern __shared__ float2 lds[];  // tile [L][S]
+  const int wr = A.wr, wc = A.wc;
+  const int img = blockIdx.x, grp = blockIdx.y;
+  const int g = threadIdx.x / TL, t = threadIdx.x - g * TL;
+  const bool live_r = g < L;           // coarse row g
+  const bool live_c = g < wc;          // window column g
+  typename PL::Tw tw;
+  PL::load_tw(tw, t, A.tw_l);
+  // column stage: value m of this thread is coarse row u = t + TL m, fed by
+  // window row kr = (u - k0r) mod L when kr < wr
+  const int kr0 = pmod(t - A.k0r, L);   // TL m < L: kr = kr0 + TL m, less L once
+  const int xcol = live_c ? pmod(A.k0c + g, L) : 0;
+  // row stage: value m is tile column x = t + TL m, holding window column
+  // (x - k0c) mod L when that is < wc
+  unsigned rmask = 0;
+#pragma unroll
+  for (int m = 0; m < VL; ++m)
+    if (live_r && pmod(t + TL * m - A.k0c, L) < wc) rmask |= 1u << m;
+  float2* row = lds + (live_r ? g : 0) * S;
+
+  float acc[VL];
+#pragma unroll
+  for (int m = 0; m < VL; ++m) acc[m] = 0.f;
+
+  for (int c = 0; c < A.nchan; ++c) {
+    const float2* sp = A.spec + ((size_t)img * A.nchan + c) * wr * wc;
+    for (int kk = grp; kk < A.nkern; kk += A.groups) {
+      const float2* kp = A.kern + (size_t)kk * wr * wc;
+      float2 v[VL];
+#pragma unroll
+      for (int m = 0; m < VL; ++m) {
+        int kr = kr0 + TL * m;
+        kr = kr >= L ? kr - L : kr;
+        v[m] = (live_c && kr < wr) ? reg::mul(kp[kr * wc + g], sp[kr * wc + g]) : make_float2(0.f, 0.f);
+      }
+      PL::template run_strided<false>(v, lds + xcol, S, t, tw, live_c);
+      if (live_c) {
+#pragma unroll
+        for (int m = 0; m < VL; ++m) lds[(t + TL * m) * S + xcol] = v[m];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int m = 0; m < VL; ++m) v[m] = ((rmask >> m) & 1u) ? row[t + TL * m] : make_float2(0.f, 0.f);
+      __syncthreads();
+      PL::template run_live<false>(v, row, t, tw, live_r);
+#pragma unroll
+      for (int m = 0; m < VL; ++m) acc[m] += v[m].x * v[m].x + v[m].y * v[m].y;
+    }
+  }
+  // epilogue: row DFT (+ sign) of coarse row g, orders q in [0, qc] -> R[img][grp][q][g]
+  const int nq = A.qc + 1;
+  float2 v[VL];
+#pragma unroll
+  for (int m = 0; m < VL; ++m) v[m] = make_float2(acc[m], 0.f);
+  PL::template run_live<true>(v, row, t, tw, live_r);
+  if (live_r) {
+    float2* dst = A.R + ((size_t)img * A.groups + grp) * nq * L + g;
+#pragma unroll
+    for (int m = 0; m < VL; ++m) {
+      const int x = t + TL * m;
+      if (x < nq) dst[(size_t)x * L] = v[m];
+    }
+  }
+}
+
 // ------------------------------------------------------- band extraction
-// Ihat[img][q][i] = scale * sum_t (sum_g R[img][g][t][q]) e^{+2 pi i qr t / L},
+// Ihat[img][q][i] = scale * sum_t (sum_g R[img][g][q][t]) e^{+2 pi i qr t / L},
 // qr = i - qr_max, i in [0, 2 qr_max + 1).
+// Small blocks (kBandBlock threads): the band has few orders per image, so
+// several blocks per image keep the grid wide.
+constexpr int kBandBlock = 64;
+
 template <class PL>
-__global__ __launch_bounds__(kBlock) void k_fp_band(const float2* __restrict__ R, int groups, int qc,
-                                                    int qrm, float scale,
-                                                    const float2* __restrict__ twt,
-                                                    float2* __restrict__ Ihat) {
-  constexpr int L = PL::N, TL = PL::T, VL = PL::V, G = groups_per_block<PL>();
+__global__ __launch_bounds__(kBandBlock) void k_fp_band(const float2* __restrict__ R, int groups,
+                                                        int qc, int qrm, float scale,
+                                                        const float2* __restrict__ twt,
+                                                        float2* __restrict__ Ihat) {
+  constexpr int L = PL::N, TL = PL::T, VL = PL::V, G = kBandBlock / TL;
+  static_assert(G >= 1, "band block holds one transform");
   extern __shared__ float2 lds[];
   const int g = threadIdx.x / TL, t = threadIdx.x - g * TL;
   const bool live = g < G;
@@ -289,12 +371,15 @@ __global__ __launch_bounds__(kBlock) void k_fp_band(const float2* __restrict__ R
 #pragma unroll
   for (int m = 0; m < VL; ++m) {
     float2 sum = make_float2(0.f, 0.f);
-    if (on)
+    if (on) {
+      const float2* src = R + ((size_t)img * groups * nq + q) * L + (t + TL * m);
+#pragma unroll 4
       for (int k = 0; k < groups; ++k) {
-        const float2 x = R[(((size_t)img * groups + k) * L + (t + TL * m)) * nq + q];
+        const float2 x = src[(size_t)k * nq * L];
         sum.x += x.x;
         sum.y += x.y;
       }
+    }
     v[m] = sum;
   }
   PL::template run<true>(v, lds + g * PL::LDS, t, tw);
@@ -394,10 +479,12 @@ __global__ __launch_bounds__(kBlock) void k_fp_urows(const float2* __restrict__ 
 // ---------------------------------------------------------------- tables
 // Image sizes: N = R1 x R2.
 #define OI_IMAGE_PLANS(X) X(128, 8, 16) X(256, 16, 16) X(512, 16, 32) X(1024, 32, 32)
-// Coarse sizes: L = R1 x R2 (column stage uses the same plan when P = 1).
+// Coarse sizes: L = R1 x R2 and the block size of the one-pass coherent kernel
+// (0: the coarse grid is too large for it; the 256-thread kernel runs).
 #define OI_COARSE_PLANS(X)                                                                      \
-  X(16, 4, 4) X(25, 5, 5) X(32, 4, 8) X(36, 6, 6) X(48, 4, 12) X(64, 8, 8) X(75, 5, 15)         \
-  X(81, 9, 9) X(100, 10, 10) X(128, 8, 16) X(144, 12, 12)
+  X(16, 4, 4, 64) X(25, 5, 5, 128) X(32, 4, 8, 128) X(36, 6, 6, 256) X(48, 4, 12, 192)         \
+  X(64, 8, 8, 512) X(75, 5, 15, 384) X(81, 9, 9, 768) X(100, 10, 10, 0)                       \
+  X(128, 8, 16, 0) X(144, 12, 12, 0)
 
 template <class P>
 size_t rows_lds() {
@@ -473,17 +560,33 @@ static int run_urows(const FpArgs& a, hipStream_t st) {
   return OI_OK;
 }
 
-template <class PL>
-static size_t coherent_lds(const FpArgs& a) {
-  using PM = PL;
-  const size_t scr = (size_t)scratch_groups<PL>() * PL::LDS;
-  const size_t scm = (size_t)scratch_groups<PM>() * PM::LDS;
-  return ((size_t)a.wr * a.wc + (size_t)PL::N * a.wc + (scr > scm ? scr : scm)) * sizeof(float2);
+// The one-pass kernel serves coarse sizes with a block size in the table when
+// the window folds onto the coarse grid without collisions.
+template <class PL, int NT>
+static bool one_pass(const FpArgs& a) {
+  return NT > 0 && a.wr <= PL::N && a.wc <= PL::N;
 }
 
-template <class PL>
+template <class PL, int NT>
+static size_t coherent_lds(const FpArgs& a) {
+  if (one_pass<PL, NT>(a)) return (size_t)PL::N * reg::padded(PL::N) * sizeof(float2);
+  const size_t scr = (size_t)scratch_groups<PL>() * PL::LDS;
+  return ((size_t)a.wr * a.wc + (size_t)PL::N * a.wc + scr) * sizeof(float2);
+}
+
+template <class PL, int NT>
 static int run_coherent(const FpArgs& a, const FpCoh& c, hipStream_t st) {
-  const size_t lds = coherent_lds<PL>(a);
+  const size_t lds = coherent_lds<PL, NT>(a);
+  if constexpr (NT > 0) {
+    if (one_pass<PL, NT>(a)) {
+      int rc = allow(k_fp_coh<PL, NT>, lds);
+      if (rc) return rc;
+      dim3 grid(a.batch, c.groups);
+      hipLaunchKernelGGL((k_fp_coh<PL, NT>), grid, dim3(NT), lds, st, c);
+      OI_LAUNCH_CHECK("fp_coh");
+      return OI_OK;
+    }
+  }
   int rc = allow(k_fp_coherent<PL, PL>, lds);
   if (rc) return rc;
   dim3 grid(1, a.batch, c.groups);
@@ -494,13 +597,13 @@ static int run_coherent(const FpArgs& a, const FpCoh& c, hipStream_t st) {
 
 template <class PL>
 static int run_band(const FpArgs& a, int groups, hipStream_t st) {
-  constexpr int G = groups_per_block<PL>();
-  const size_t lds = rows_lds<PL>();
+  constexpr int G = kBandBlock / PL::T;
+  const size_t lds = (size_t)((kBandBlock + PL::T - 1) / PL::T) * PL::LDS * sizeof(float2);
   int rc = allow(k_fp_band<PL>, lds);
   if (rc) return rc;
   dim3 grid((a.qc + 1 + G - 1) / G, a.batch);
   const float scale = 1.0f / ((float)PL::N * (float)PL::N);
-  hipLaunchKernelGGL(k_fp_band<PL>, grid, dim3(kBlock), lds, st, a.R, groups, a.qc, a.qr, scale,
+  hipLaunchKernelGGL(k_fp_band<PL>, grid, dim3(kBandBlock), lds, st, a.R, groups, a.qc, a.qr, scale,
                      a.tw_l, a.Ihat);
   OI_LAUNCH_CHECK("fp_band");
   return OI_OK;
@@ -517,7 +620,7 @@ bool fp_image_size_ok(int n) {
 }
 
 bool fp_coarse_size_ok(int l) {
-#define X(NN, A, B) if (l == NN) return true;
+#define X(NN, A, B, NT) if (l == NN) return true;
   OI_COARSE_PLANS(X)
 #undef X
   return false;
@@ -525,7 +628,7 @@ bool fp_coarse_size_ok(int l) {
 
 int fp_coarse_choose(int need) {
   int best = 0;
-#define X(NN, A, B) if (NN >= need && (best == 0 || NN < best)) best = NN;
+#define X(NN, A, B, NT) if (NN >= need && (best == 0 || NN < best)) best = NN;
   OI_COARSE_PLANS(X)
 #undef X
   return best;
@@ -535,19 +638,19 @@ size_t fp_coherent_lds(int L, int wr, int wc) {
   FpArgs a{};
   a.wr = wr;
   a.wc = wc;
-#define X(NN, A, B) if (L == NN) return fp::coherent_lds<reg::GFFT<NN, A, B>>(a);
+#define X(NN, A, B, NT) if (L == NN) return fp::coherent_lds<reg::GFFT<NN, A, B>, NT>(a);
   OI_COARSE_PLANS(X)
 #undef X
   return (size_t)-1;
 }
 
 int fp_coherent_rows_fit(int L) {
-  // accumulator registers per thread: ceil(L / groups) * V
-#define X(NN, A, B)                                                                      \
+  // accumulator registers per thread of the 256-thread kernel: ceil(L / groups) * V
+#define X(NN, A, B, NT)                                                                  \
   if (L == NN) {                                                                         \
     using P = reg::GFFT<NN, A, B>;                                                       \
     constexpr int G = fp::groups_per_block<P>();                                         \
-    return ((NN + G - 1) / G) * P::V <= 96;                                              \
+    return (NT > 0) || ((NN + G - 1) / G) * P::V <= 96;                                  \
   }
   OI_COARSE_PLANS(X)
 #undef X
@@ -571,13 +674,13 @@ int fp_spectrum(const FpArgs& a, hipStream_t st) {
 
 int fp_image(const FpArgs& a, const FpCoh& c, hipStream_t st) {
   int rc = OI_ERR_UNSUPPORTED;
-#define X(NN, A, B) if (a.L == NN) rc = fp::run_coherent<reg::GFFT<NN, A, B>>(a, c, st);
+#define X(NN, A, B, NT) if (a.L == NN) rc = fp::run_coherent<reg::GFFT<NN, A, B>, NT>(a, c, st);
   OI_COARSE_PLANS(X)
 #undef X
   if (rc) return rc == OI_ERR_UNSUPPORTED ? fail(rc, "fast path: unsupported coarse size") : rc;
   if (a.mark) a.mark(a.mark_ctx, 3);
   rc = OI_ERR_UNSUPPORTED;
-#define X(NN, A, B) if (a.L == NN) rc = fp::run_band<reg::GFFT<NN, A, B>>(a, c.groups, st);
+#define X(NN, A, B, NT) if (a.L == NN) rc = fp::run_band<reg::GFFT<NN, A, B>>(a, c.groups, st);
   OI_COARSE_PLANS(X)
 #undef X
   if (rc) return rc;

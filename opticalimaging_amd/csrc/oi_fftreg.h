@@ -205,6 +205,58 @@ struct GFFT {
   // every thread of the block must call (barriers inside) when R2 > 1.
   template <bool INV>
   __device__ __forceinline__ static void run(float2 (&v)[V], float2* lds, int t, const Tw& tw) {
+    run_impl<INV>(v, [lds](int i) -> float2& { return lds[pad(i)]; }, t, tw, true);
+  }
+
+  // The same transform exchanging through base[i * stride], i in [0, N) (no
+  // padding); threads with live == false take part in the barriers only.
+  template <bool INV>
+  __device__ __forceinline__ static void run_strided(float2 (&v)[V], float2* base, int stride, int t,
+                                                     const Tw& tw, bool live) {
+    run_impl<INV>(v, [base, stride](int i) -> float2& { return base[i * stride]; }, t, tw, live);
+  }
+
+  // Exchange through a padded contiguous region (pad(i) < padded(N)), live-gated.
+  template <bool INV>
+  __device__ __forceinline__ static void run_live(float2 (&v)[V], float2* lds, int t, const Tw& tw,
+                                                  bool live) {
+    run_impl<INV>(v, [lds](int i) -> float2& { return lds[pad(i)]; }, t, tw, live);
+  }
+
+  // Twiddles read from a table at use (table[m] = exp(-2 pi i m / N), in LDS
+  // or global memory) instead of registers.
+  struct TabTw {
+    const float2* table;
+    __device__ __forceinline__ float2 get(int u, int j, int t) const {
+      return table[j * ((t + T * u) % R1)];
+    }
+  };
+  struct RegTw {
+    const Tw& tw;
+    __device__ __forceinline__ float2 get(int u, int j, int) const { return tw.w[u][j - 1]; }
+  };
+
+  template <bool INV>
+  __device__ __forceinline__ static void run_strided_tab(float2 (&v)[V], float2* base, int stride,
+                                                         int t, const float2* table, bool live) {
+    run_gen<INV>(v, [base, stride](int i) -> float2& { return base[i * stride]; }, t,
+                 TabTw{table}, live);
+  }
+  template <bool INV>
+  __device__ __forceinline__ static void run_live_tab(float2 (&v)[V], float2* lds, int t,
+                                                      const float2* table, bool live) {
+    run_gen<INV>(v, [lds](int i) -> float2& { return lds[pad(i)]; }, t, TabTw{table}, live);
+  }
+
+  template <bool INV, class Addr>
+  __device__ __forceinline__ static void run_impl(float2 (&v)[V], Addr at, int t, const Tw& tw,
+                                                  bool live) {
+    run_gen<INV>(v, at, t, RegTw{tw}, live);
+  }
+
+  template <bool INV, class Addr, class TW>
+  __device__ __forceinline__ static void run_gen(float2 (&v)[V], Addr at, int t, const TW& tw,
+                                                 bool live) {
     if constexpr (R2 == 1) {
       DFT<N, INV>::run(v);
     } else {
@@ -215,8 +267,10 @@ struct GFFT {
         for (int j = 0; j < R1; ++j) b[j] = v[u + j * U1];
         DFT<R1, INV>::run(b);
         const int i = t + T * u;
+        if (live) {
 #pragma unroll
-        for (int j = 0; j < R1; ++j) lds[pad(i * R1 + j)] = b[j];
+          for (int j = 0; j < R1; ++j) at(i * R1 + j) = b[j];
+        }
       }
       __syncthreads();
 #pragma unroll
@@ -224,10 +278,10 @@ struct GFFT {
         const int i = t + T * u;
         float2 b[R2];
 #pragma unroll
-        for (int j = 0; j < R2; ++j) b[j] = lds[pad(i + j * (N / R2))];
+        for (int j = 0; j < R2; ++j) b[j] = at(i + j * (N / R2));
 #pragma unroll
         for (int j = 1; j < R2; ++j) {
-          const float2 w = tw.w[u][j - 1];
+          const float2 w = tw.get(u, j, t);
           b[j] = mul(b[j], INV ? make_float2(w.x, -w.y) : w);
         }
         DFT<R2, INV>::run(b);

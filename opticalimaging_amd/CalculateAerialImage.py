@@ -10,6 +10,9 @@ pupil (2 floor(2 p) + 1) and Fourier-interpolated exactly to the image grid.
 
 ``CalculateHopkinsImage``: per focus, TCC -> SOCS kernels -> SOCS image
 (CalculateAerialImage.py:105-130).
+
+``CalculateOptimized``: per focus, TCC -> SOCS kernels on the Floquet-order
+grid -> ``CalculateSOCS`` on the ScatterGrid (CalculateAerialImage.py:131-152).
 """
 from __future__ import annotations
 
@@ -17,13 +20,13 @@ import torch
 
 from . import _native as nat
 from . import engine, optics
-from .CalculateAerialImage_SOCS import CalculateAerialImage_SOCS
+from .CalculateAerialImage_SOCS import CalculateAerialImage_SOCS, CalculateSOCS
 from .CalculateTCCMatrix import CalculateTCCMatrix
 from .DecomposeTCC_SOCS import DecomposeTCC_SOCS
 from .Scatter import _pitches, pupil_image
 
 __all__ = ["ImageData", "cartesian_to_polar", "CalculateAbbeImage", "CalculateHopkinsImage",
-           "abbe_images"]
+           "CalculateOptimized", "abbe_images"]
 
 
 class ImageData:
@@ -107,3 +110,28 @@ def CalculateHopkinsImage(source, scatter, projector, numerics):
     d = _image_axes(scatter, nx, ny, projector)
     d.Intensity = intensity.to(h.device)
     return d, pupil
+
+
+def CalculateOptimized(source, scatter, projector, numerics):
+    """Returns (ImageData, empty tensor) (CalculateAerialImage.py:131-152).
+
+    scatter.FloquetMode: complex (2 ScatterOrder_X + 1, 2 ScatterOrder_Y + 1, 3)
+    Floquet amplitudes of the near field; the kernels of each focus live on
+    that order grid (DecomposeTCC_SOCS with Nfg = FloquetMode.shape[:2])."""
+    fm = torch.as_tensor(scatter.FloquetMode)
+    pitchxy = [scatter.Period_Y, scatter.Period_X]
+    nfg = [int(fm.shape[0]), int(fm.shape[1])]
+    foci = optics.focus_values(projector)
+    gx, gy = int(numerics.ScatterGrid_X), int(numerics.ScatterGrid_Y)
+    intensity = torch.zeros((len(foci), gx, gy))
+    for i, z in enumerate(foci):
+        tcc, fg = CalculateTCCMatrix(source, pitchxy, projector, z, numerics)
+        kern = DecomposeTCC_SOCS(tcc, fg, nfg, numerics)
+        intensity[i] = CalculateSOCS(fm, kern, numerics).cpu()
+        del tcc
+    d = ImageData()
+    d.Intensity = intensity.to(fm.device)
+    d.ImageX = torch.linspace(-scatter.Period_X / 2, scatter.Period_X / 2, gx)
+    d.ImageY = torch.linspace(-scatter.Period_Y / 2, scatter.Period_Y / 2, gy)
+    d.ImageZ = projector.FocusRange
+    return d, torch.tensor([])

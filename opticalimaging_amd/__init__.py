@@ -18,11 +18,11 @@ from .Scatter import Scatter
 from .CalculateCharacteristicMatrix import CalculateCharacteristicMatrix
 from .CalculateTCCMatrix import CalculateTCCMatrix, GetTCCMatrix
 from .DecomposeTCC_SOCS import DecomposeTCC_SOCS
-from .CalculateAerialImage_SOCS import CalculateAerialImage_SOCS
-from .CalculateAerialImage import ImageData, CalculateAbbeImage, CalculateHopkinsImage
+from .CalculateAerialImage_SOCS import CalculateAerialImage_SOCS, CalculateSOCS
+from .CalculateAerialImage import ImageData, CalculateAbbeImage, CalculateHopkinsImage, CalculateOptimized
 from .engine import KernelSet, socs_image
 
 __all__ = ["Numerics", "Source", "SourceData", "PolarizationParameters", "Projection", "Scatter",
            "CalculateCharacteristicMatrix", "CalculateTCCMatrix", "GetTCCMatrix",
            "DecomposeTCC_SOCS", "CalculateAerialImage_SOCS", "ImageData", "CalculateAbbeImage",
-           "CalculateHopkinsImage", "KernelSet", "socs_image"]
+           "CalculateHopkinsImage", "CalculateOptimized", "CalculateSOCS", "KernelSet", "socs_image"]

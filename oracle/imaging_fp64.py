@@ -23,6 +23,9 @@ Pins (tests/test_oracle_*.py):
                              Gram-route equivalence.
   * socs_image            -- full-order SOCS equals Abbe (even grids),
                              clear-field value.
+  * calculate_socs        -- one order: constant |FM phi|^2; two orders: the
+                             closed-form fringe (sign, axis order, half-pixel
+                             offset); channel and kernel sums.
 """
 from __future__ import annotations
 
@@ -37,6 +40,7 @@ __all__ = [
     "shifted_pupil_matrix", "get_tcc_matrix", "tcc_matrix", "eig_desc",
     "socs_count", "decompose_tcc_socs", "decompose_lowrank", "embed_kernels",
     "socs_image", "hopkins_image", "dft2_bruteforce", "clear_field_intensity",
+    "calculate_socs", "optimized_image",
 ]
 
 
@@ -566,3 +570,55 @@ def dft2_bruteforce(x, sign):
     wy = np.exp(sign * 2j * np.pi * np.outer(np.arange(ny), np.arange(ny)) / ny)
     wx = np.exp(sign * 2j * np.pi * np.outer(np.arange(nx), np.arange(nx)) / nx)
     return np.einsum("ab,bc...->ac...", wy, np.einsum("cd,bd...->bc...", wx, x))
+
+
+# ---------------------------------------------------------------------------
+# Floquet-mode SOCS (CalculateAerialImage_SOCS.py:13-32, CalculateAerialImage.py:131-152)
+# ---------------------------------------------------------------------------
+
+def calculate_socs(floquet_mode, kernels, grid_x, grid_y, order_x, order_y):
+    """Direct float64 sum of CalculateSOCS (CalculateAerialImage_SOCS.py:13-32):
+    S[x, y, c, k] = sum_o exp(2 pi i (ox (ix + 1/2)/Gx + oy (iy + 1/2)/Gy))
+    FM[o, c] phi[o, k]; I[x, y] = sum_{c,k} |S|^2.  Orders o run over the
+    'ij' meshgrid of [-Ox, Ox] x [-Oy, Oy] flattened row-major; phi[o] is the
+    kernel array transposed (dims 0, 1) and ifftshifted, flattened the same way.
+    Returns (Gx, Gy)."""
+    fm = np.asarray(floquet_mode).astype(np.complex128).reshape(-1, 3)
+    K = kernels.shape[2]
+    kt = np.fft.ifftshift(np.transpose(np.asarray(kernels).astype(np.complex128), (1, 0, 2)),
+                          axes=(0, 1)).reshape(-1, K)
+    ox = np.arange(-order_x, order_x + 1)
+    oy = np.arange(-order_y, order_y + 1)
+    gx, gy = np.meshgrid(ox, oy, indexing="ij")
+    ox_d, oy_d = gx.reshape(-1), gy.reshape(-1)
+    x = (np.arange(grid_x) + 0.5) / grid_x
+    y = (np.arange(grid_y) + 0.5) / grid_y
+    ex = np.exp(2j * np.pi * np.outer(x, ox_d))            # (Gx, O)
+    ey = np.exp(2j * np.pi * np.outer(y, oy_d))            # (Gy, O)
+    out = np.zeros((grid_x, grid_y))
+    for c in range(3):
+        for k in range(K):
+            a = fm[:, c] * kt[:, k]                          # (O,)
+            S = np.einsum("xo,yo,o->xy", ex, ey, a)
+            out += S.real ** 2 + S.imag ** 2
+    return out
+
+
+def optimized_image(src, scatter, proj, numerics):
+    """CalculateOptimized (CalculateAerialImage.py:131-152), float64: per focus
+    TCC -> SOCS kernels on the Floquet grid -> calculate_socs.  Returns
+    (nFocus, ScatterGrid_X, ScatterGrid_Y)."""
+    fm = scatter.FloquetMode
+    if isinstance(fm, torch.Tensor):
+        fm = fm.detach().cpu().numpy()
+    fm = np.asarray(fm)
+    pitch = [scatter.Period_Y, scatter.Period_X]
+    nfg = [fm.shape[0], fm.shape[1]]
+    foci = _foci(proj)
+    out = np.zeros((len(foci), numerics.ScatterGrid_X, numerics.ScatterGrid_Y))
+    for fi, z in enumerate(foci):
+        tcc, fg = tcc_matrix(src, pitch, proj, z, numerics)
+        kern = decompose_tcc_socs(tcc, fg, nfg, numerics)
+        out[fi] = calculate_socs(fm, kern, numerics.ScatterGrid_X, numerics.ScatterGrid_Y,
+                                 numerics.ScatterOrder_X, numerics.ScatterOrder_Y)
+    return out

@@ -410,3 +410,63 @@ def test_autograd_gradient_matches_central_differences():
         mm[y, x] -= eps
         fd = (loss(mp) - loss(mm)) / (2 * eps)
         assert abs(fd - grad[y, x]) <= 1e-9 * np.max(np.abs(grad)), (y, x, fd, grad[y, x])
+
+
+# ------------------------------------------------------- Floquet-mode SOCS pins
+def _floquet_kernel(n0, n1, K, entries):
+    """Kernels in DecomposeTCC_SOCS's layout (transposed, fftshifted) holding
+    entries {(ox, oy, k): value} at Floquet orders (ox, oy)."""
+    kc = np.zeros((n0, n1, K), dtype=np.complex128)
+    for (ox, oy, k), v in entries.items():
+        kc[ox + n0 // 2, oy + n1 // 2, k] = v
+    return np.transpose(np.fft.fftshift(kc, axes=(0, 1)), (1, 0, 2))
+
+
+def test_calculate_socs_single_order_constant():
+    Ox, Oy = 2, 3
+    n0, n1 = 2 * Ox + 1, 2 * Oy + 1
+    fm = np.zeros((n0, n1, 3), dtype=np.complex128)
+    fm[Ox + 1, Oy - 2, 1] = 0.5 - 1.5j
+    kern = _floquet_kernel(n0, n1, 2, {(1, -2, 1): 2.0 + 1.0j})
+    I = O.calculate_socs(fm, kern, 9, 7, Ox, Oy)
+    expect = abs((0.5 - 1.5j) * (2.0 + 1.0j)) ** 2
+    assert I.shape == (9, 7)
+    assert np.allclose(I, expect, rtol=1e-13)
+
+
+def test_calculate_socs_two_order_fringe_closed_form():
+    """I = |a1|^2 + |a2|^2 + 2 |a1 a2| cos(2 pi (d_x (ix + 1/2)/Gx + d_y (iy + 1/2)/Gy) + arg a1 - arg a2),
+    d = o1 - o2: pins the phase sign, which array axis carries order_x, and
+    the half-pixel sample offset."""
+    Ox, Oy, Gx, Gy = 3, 2, 11, 8
+    n0, n1 = 2 * Ox + 1, 2 * Oy + 1
+    o1, o2 = (2, -1), (-1, 1)
+    f1, f2 = 0.7 + 0.2j, -0.3 + 0.9j
+    p1, p2 = 1.1 - 0.4j, 0.6 + 0.5j
+    fm = np.zeros((n0, n1, 3), dtype=np.complex128)
+    fm[o1[0] + Ox, o1[1] + Oy, 2] = f1
+    fm[o2[0] + Ox, o2[1] + Oy, 2] = f2
+    kern = _floquet_kernel(n0, n1, 1, {(o1[0], o1[1], 0): p1, (o2[0], o2[1], 0): p2})
+    I = O.calculate_socs(fm, kern, Gx, Gy, Ox, Oy)
+    a1, a2 = f1 * p1, f2 * p2
+    ix = (np.arange(Gx)[:, None] + 0.5) / Gx
+    iy = (np.arange(Gy)[None, :] + 0.5) / Gy
+    ph = 2 * np.pi * ((o1[0] - o2[0]) * ix + (o1[1] - o2[1]) * iy) + np.angle(a1) - np.angle(a2)
+    expect = abs(a1) ** 2 + abs(a2) ** 2 + 2 * abs(a1 * a2) * np.cos(ph)
+    assert np.max(np.abs(I - expect)) < 1e-12
+
+
+def test_calculate_socs_sums_channels_and_kernels():
+    rng = np.random.default_rng(21)
+    Ox, Oy = 2, 2
+    n = 2 * Ox + 1
+    fm = rng.standard_normal((n, n, 3)) + 1j * rng.standard_normal((n, n, 3))
+    kern = rng.standard_normal((n, n, 3)) + 1j * rng.standard_normal((n, n, 3))
+    total = O.calculate_socs(fm, kern, 6, 10, Ox, Oy)
+    parts = 0
+    for c in range(3):
+        for k in range(3):
+            f1 = np.zeros_like(fm)
+            f1[..., c] = fm[..., c]
+            parts = parts + O.calculate_socs(f1, kern[..., k:k + 1], 6, 10, Ox, Oy)
+    assert np.max(np.abs(total - parts)) < 1e-11 * np.max(total)

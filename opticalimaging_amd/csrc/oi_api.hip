@@ -102,6 +102,8 @@ struct oi_plan {
   float2* U = nullptr;
   float* Ic = nullptr;
   float2* Ihat = nullptr;
+  float2* Bbuf = nullptr;   // two-pass column fields, grown to the largest kernel count used
+  size_t Bcap = 0;
   size_t bytes = 0;
   std::vector<hipEvent_t> ev;  // slots * (kStages + 1)
   int slots = 0, next = 0, recorded = 0;
@@ -126,7 +128,7 @@ static void plan_free(oi_plan* p) {
   if (!p) return;
   for (hipEvent_t e : p->ev) (void)hipEventDestroy(e);
   for (void* ptr : {(void*)p->T, (void*)p->spec, (void*)p->R, (void*)p->U, (void*)p->Ic,
-                    (void*)p->Ihat})
+                    (void*)p->Ihat, (void*)p->Bbuf})
     if (ptr) (void)hipFree(ptr);
   free_twiddles(p->pool);
   delete p;
@@ -178,6 +180,18 @@ extern "C" int oi_plan_create(oi_geometry* geo, oi_plan** out) {
         fp_coherent_lds(L, g.wr, g.wc) <= kLdsLimit) {
       g.path = 1;
       g.lr = g.lc = lr = lc = L;
+    }
+  }
+  // Two-pass path: coarse grids beyond one workgroup's reach (L = 256, 512)
+  // when the window folds onto them without collisions.
+  if (g.path == 0 && !g.direct && geo->lr <= 0 && geo->lc <= 0) {
+    const int need = (2 * band_r - 1) > (2 * band_c - 1) ? 2 * band_r - 1 : 2 * band_c - 1;
+    const int L = need > 144 ? fp_two_pass_choose(need) : 0;
+    if (L > 0 && L < g.ny && L < g.nx && g.wr <= L && g.wc <= L &&
+        fp_two_pass_lds(L, g.wc) <= kLdsLimit) {
+      g.path = 2;
+      g.lr = g.lc = lr = lc = L;
+      if (g.max_groups <= 0) g.max_groups = 1;
     }
   }
   // Coarse rows per workgroup: largest of 16, 8, 4, 2, 1 dividing lr whose
@@ -323,7 +337,7 @@ static int spectrum_impl(oi_plan* p, const void* field, int kind, int cin, const
   int rc;
   const oi_geometry& g = p->g;
   mark(p, 0, st);
-  if (g.path) {
+  if (g.path == 1 || (g.path == 2 && fp_image_size_ok(g.ny) && fp_image_size_ok(g.nx))) {
     FpArgs a = fast_args(p, batch, st);
     a.field = field;
     a.kind = kind;
@@ -353,6 +367,63 @@ extern "C" int oi_coherent_image(oi_plan* p, const void* spec, int batch, const 
   const oi_geometry& g = p->g;
   const int phases = g.lr / g.rows;
   int groups = g.max_groups < nkern ? g.max_groups : nkern;
+  if (g.path == 2) {
+    const size_t need = (size_t)batch * nkern * g.nchan * g.wc * g.lr * sizeof(float2);
+    if (need > p->Bcap) {   // first call with this many kernels: grow the column-field buffer
+      if (p->Bbuf) OI_HIP_CHECK(hipFree(p->Bbuf));
+      p->Bbuf = nullptr;
+      p->bytes -= p->Bcap;
+      p->Bcap = 0;
+      if (hipMalloc((void**)&p->Bbuf, need) != hipSuccess) {
+        p->Bbuf = nullptr;
+        return fail(OI_ERR_NOMEM, "coherent_image: column-field buffer allocation failed");
+      }
+      p->Bcap = need;
+      p->bytes += need;
+    }
+    FpArgs a = fast_args(p, batch, st);
+    a.out = out;
+    FpCoh c{};
+    c.spec = static_cast<const float2*>(spec);
+    c.kern = static_cast<const float2*>(kern);
+    c.nchan = g.nchan;
+    c.nkern = nkern;
+    c.k0r = g.k0r;
+    c.wr = g.wr;
+    c.k0c = g.k0c;
+    c.wc = g.wc;
+    c.groups = 1;
+    c.qc = g.qc;
+    c.tw_l = p->flr.tw;
+    c.tw_m = p->flr.tw;
+    c.R = p->R;
+    const bool fast_io = fp_image_size_ok(g.ny) && fp_image_size_ok(g.nx);
+    int rc = fp_two_pass(a, c, p->Bbuf, fast_io ? 0 : 1, st);
+    if (rc) return rc;
+    mark(p, 3, st);
+    if (fast_io) {
+      rc = fp_interp(a, 1, st);
+    } else {
+      InterpArgs ia{};
+      ia.R = p->R;
+      ia.groups = 1;
+      ia.lr = g.lr;
+      ia.lc = g.lc;
+      ia.ny = g.ny;
+      ia.nx = g.nx;
+      ia.qr = g.qr;
+      ia.qc = g.qc;
+      ia.scale = 1.0f / ((float)g.lr * (float)g.lc);
+      ia.flr = p->flr;
+      ia.fny = p->fy;
+      ia.fnx = p->fx;
+      ia.U = p->U;
+      ia.out = out;
+      rc = launch_interp(ia, batch, st, mark_cb, &g_mark_ctx);
+    }
+    mark_done(p);
+    return rc;
+  }
   if (g.path) {
     FpArgs a = fast_args(p, batch, st);
     a.out = out;

@@ -50,5 +50,13 @@ size_t fp_coherent_lds(int L, int wr, int wc);
 int fp_coherent_rows_fit(int L);
 int fp_spectrum(const FpArgs& a, hipStream_t st);  // marks stage 1 between its launches
 int fp_image(const FpArgs& a, const FpCoh& c, hipStream_t st);  // marks stages 3, 4, 5
+int fp_interp(const FpArgs& a, int groups, hipStream_t st);      // marks stages 4, 5
+// Two-pass coherent sum for coarse grids L in {256, 512}: B holds
+// batch * nkern * nchan * wc * L c64; R in [img][q][t] (generic_layout = 0) or
+// [img][t][q] (1).
+bool fp_two_pass_size_ok(int l);
+int fp_two_pass_choose(int need);
+size_t fp_two_pass_lds(int L, int wc);
+int fp_two_pass(const FpArgs& a, const FpCoh& c, float2* B, int generic_layout, hipStream_t st);
 
 }  // namespace oi

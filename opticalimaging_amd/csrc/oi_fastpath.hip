@@ -267,31 +267,34 @@ __global__ __launch_bounds__(kBlock) void k_fp_coherent(FpCoh A) {
 }
 
 // Coherent sum, one workgroup per (image, kernel group), every coarse row and
-// every window column in one pass: NT = (groups of T_L threads) covers L rows.
-// All transforms exchange in place through one LDS tile [L][S] (S = padded L):
-// the column stage of window column kc works in tile column (k0c + kc) mod L,
-// the row stage of coarse row a in tile row a.  Requires wr, wc <= L (the
-// window folds onto the coarse grid without collisions).
+// every window column in one pass.  Groups of T_L lanes never straddle a
+// wavefront (GW = 64 / T_L groups per wave, NT / 64 waves, NT/64 * GW >= L), so
+// each transform exchanges with wave-level ordering only.  All transforms work
+// in place in one LDS tile [L][S] (S = padded L): the column stage of window
+// column kc in tile column (k0c + kc) mod L, the row stage of coarse row a in
+// tile row a.  Two workgroup barriers per kernel: after the column stage (rows
+// read every column) and after the row stage (the next column stage writes
+// across rows).  Requires wr, wc <= L (no fold collisions).
 template <class PL, int NT>
 __global__ __launch_bounds__(NT) void k_fp_coh(FpCoh A) {
   constexpr int L = PL::N, TL = PL::T, VL = PL::V;
   constexpr int S = reg::padded(L);
-  static_assert(L * TL <= NT, "one group per coarse row");
+  constexpr int GW = 64 / TL;
+  static_assert(GW >= 1 && (NT / 64) * GW >= L, "one group per coarse row");
   static_assert(VL <= 32, "row mask holds VL bits");
   extern __shared__ float2 lds[];  // tile [L][S]
   const int wr = A.wr, wc = A.wc;
   const int img = blockIdx.x, grp = blockIdx.y;
-  const int g = threadIdx.x / TL, t = threadIdx.x - g * TL;
-  const bool live_r = g < L;           // coarse row g
-  const bool live_c = g < wc;          // window column g
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int gw = lane / TL, t = lane - gw * TL;
+  const int g = wave * GW + gw;
+  const bool lane_ok = gw < GW;
+  const bool live_r = lane_ok && g < L;   // coarse row g
+  const bool live_c = lane_ok && g < wc;  // window column g
   typename PL::Tw tw;
   PL::load_tw(tw, t, A.tw_l);
-  // column stage: value m of this thread is coarse row u = t + TL m, fed by
-  // window row kr = (u - k0r) mod L when kr < wr
-  const int kr0 = pmod(t - A.k0r, L);   // TL m < L: kr = kr0 + TL m, less L once
+  const int kr0 = pmod(t - A.k0r, L);     // TL m < L: kr = kr0 + TL m, less L once
   const int xcol = live_c ? pmod(A.k0c + g, L) : 0;
-  // row stage: value m is tile column x = t + TL m, holding window column
-  // (x - k0c) mod L when that is < wc
   unsigned rmask = 0;
 #pragma unroll
   for (int m = 0; m < VL; ++m)
@@ -313,7 +316,7 @@ __global__ __launch_bounds__(NT) void k_fp_coh(FpCoh A) {
         kr = kr >= L ? kr - L : kr;
         v[m] = (live_c && kr < wr) ? reg::mul(kp[kr * wc + g], sp[kr * wc + g]) : make_float2(0.f, 0.f);
       }
-      PL::template run_strided<false>(v, lds + xcol, S, t, tw, live_c);
+      PL::template run_strided_wave<false>(v, lds + xcol, S, t, tw, live_c);
       if (live_c) {
 #pragma unroll
         for (int m = 0; m < VL; ++m) lds[(t + TL * m) * S + xcol] = v[m];
@@ -321,10 +324,11 @@ __global__ __launch_bounds__(NT) void k_fp_coh(FpCoh A) {
       __syncthreads();
 #pragma unroll
       for (int m = 0; m < VL; ++m) v[m] = ((rmask >> m) & 1u) ? row[t + TL * m] : make_float2(0.f, 0.f);
-      __syncthreads();
-      PL::template run_live<false>(v, row, t, tw, live_r);
+      reg::wave_sync();  // the group's loads precede its in-place exchange
+      PL::template run_live_wave<false>(v, row, t, tw, live_r);
 #pragma unroll
       for (int m = 0; m < VL; ++m) acc[m] += v[m].x * v[m].x + v[m].y * v[m].y;
+      __syncthreads();
     }
   }
   // epilogue: row DFT (+ sign) of coarse row g, orders q in [0, qc] -> R[img][grp][q][g]
@@ -332,7 +336,7 @@ __global__ __launch_bounds__(NT) void k_fp_coh(FpCoh A) {
   float2 v[VL];
 #pragma unroll
   for (int m = 0; m < VL; ++m) v[m] = make_float2(acc[m], 0.f);
-  PL::template run_live<true>(v, row, t, tw, live_r);
+  PL::template run_live_wave<true>(v, row, t, tw, live_r);
   if (live_r) {
     float2* dst = A.R + ((size_t)img * A.groups + grp) * nq * L + g;
 #pragma unroll
@@ -340,6 +344,101 @@ __global__ __launch_bounds__(NT) void k_fp_coh(FpCoh A) {
       const int x = t + TL * m;
       if (x < nq) dst[(size_t)x * L] = v[m];
     }
+  }
+}
+
+// ------------------------------------------------ two-pass coherent sum
+// For coarse grids too large for one workgroup (L = 256, 512).  Pass 1
+// transforms every window column of every (kernel, channel) into
+// B[img][kc][c][t] (column-major, t the coarse row); pass 2 gives each block G
+// coarse rows, loops over all (kernel, channel) pairs reading its rows of B
+// through an LDS tile, and accumulates |E|^2 in registers; its epilogue takes
+// the row DFT (+ sign) to R.  Stage twiddles are read from the table at use.
+template <class PL>
+__global__ __launch_bounds__(kBlock) void k_fp_bcols(const float2* __restrict__ spec,
+                                                     const float2* __restrict__ kern, int nk,
+                                                     int nchan, int k0r, int wr, int wc,
+                                                     const float2* __restrict__ twt,
+                                                     float2* __restrict__ B) {
+  constexpr int L = PL::N, TL = PL::T, VL = PL::V, G = groups_per_block<PL>();
+  extern __shared__ float2 lds[];
+  const int g = threadIdx.x / TL, t = threadIdx.x - g * TL;
+  const int c = blockIdx.x * G + g;
+  const int kc = blockIdx.y, img = blockIdx.z;        // kc = k * nchan + channel
+  const int k = kc / nchan, ch = kc - k * nchan;
+  const bool live = g < G && c < wc;
+  const float2* kp = kern + (size_t)k * wr * wc;
+  const float2* sp = spec + ((size_t)img * nchan + ch) * wr * wc;
+  const int kr0 = pmod(t - k0r, L);
+  float2 v[VL];
+#pragma unroll
+  for (int m = 0; m < VL; ++m) {
+    int kr = kr0 + TL * m;
+    kr = kr >= L ? kr - L : kr;
+    v[m] = (live && kr < wr) ? reg::mul(kp[kr * wc + c], sp[kr * wc + c]) : make_float2(0.f, 0.f);
+  }
+  PL::template run_live_tab<false>(v, lds + g * PL::LDS, t, twt, live);
+  if (!live) return;
+  float2* dst = B + (((size_t)img * nk * nchan + kc) * wc + c) * L;
+#pragma unroll
+  for (int m = 0; m < VL; ++m) dst[t + TL * m] = v[m];
+}
+
+template <class PL>
+__global__ __launch_bounds__(kBlock) void k_fp_brows(const float2* __restrict__ B, int nkc, int k0c,
+                                                     int wc, int qc, int generic_layout,
+                                                     const float2* __restrict__ twt,
+                                                     float2* __restrict__ R) {
+  constexpr int L = PL::N, TL = PL::T, VL = PL::V, G = groups_per_block<PL>();
+  constexpr int TS = G + 1;                // tile row stride (bank spread)
+  static_assert(VL <= 32, "row mask holds VL bits");
+  extern __shared__ float2 lds[];
+  float2* tile = lds;                      // [wc][TS]
+  float2* scr = lds + (size_t)wc * TS;     // transform scratch
+  const int g = threadIdx.x / TL, t = threadIdx.x - g * TL;
+  const int img = blockIdx.y;
+  const int r0 = blockIdx.x * G;
+  const int row = r0 + g;
+  const bool live = g < G && row < L;
+  unsigned rmask = 0;
+  int cidx[VL];
+#pragma unroll
+  for (int m = 0; m < VL; ++m) {
+    const int cc = pmod(t + TL * m - k0c, L);
+    cidx[m] = cc < wc ? cc : 0;
+    if (live && cc < wc) rmask |= 1u << m;
+  }
+  float acc[VL];
+#pragma unroll
+  for (int m = 0; m < VL; ++m) acc[m] = 0.f;
+  for (int kc = 0; kc < nkc; ++kc) {
+    const float2* src = B + ((size_t)img * nkc + kc) * wc * L + r0;
+    __syncthreads();
+    for (int e = threadIdx.x; e < wc * G; e += kBlock) {
+      const int cc = e / G, r = e - cc * G;
+      tile[cc * TS + r] = (r0 + r < L) ? src[(size_t)cc * L + r] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    float2 v[VL];
+#pragma unroll
+    for (int m = 0; m < VL; ++m)
+      v[m] = ((rmask >> m) & 1u) ? tile[cidx[m] * TS + (g < G ? g : 0)] : make_float2(0.f, 0.f);
+    PL::template run_live_tab<false>(v, scr + g * PL::LDS, t, twt, g < G);
+#pragma unroll
+    for (int m = 0; m < VL; ++m) acc[m] += v[m].x * v[m].x + v[m].y * v[m].y;
+  }
+  float2 v[VL];
+#pragma unroll
+  for (int m = 0; m < VL; ++m) v[m] = make_float2(acc[m], 0.f);
+  PL::template run_live_tab<true>(v, scr + g * PL::LDS, t, twt, g < G);
+  if (!live) return;
+  const int nq = qc + 1;
+#pragma unroll
+  for (int m = 0; m < VL; ++m) {
+    const int x = t + TL * m;
+    if (x > qc) continue;
+    if (generic_layout) R[((size_t)img * L + row) * nq + x] = v[m];   // [img][t][q] (one group)
+    else R[((size_t)img * nq + x) * L + row] = v[m];                   // [img][q][t]
   }
 }
 
@@ -482,8 +581,8 @@ __global__ __launch_bounds__(kBlock) void k_fp_urows(const float2* __restrict__ 
 // Coarse sizes: L = R1 x R2 and the block size of the one-pass coherent kernel
 // (0: the coarse grid is too large for it; the 256-thread kernel runs).
 #define OI_COARSE_PLANS(X)                                                                      \
-  X(16, 4, 4, 64) X(25, 5, 5, 128) X(32, 4, 8, 128) X(36, 6, 6, 256) X(48, 4, 12, 192)         \
-  X(64, 8, 8, 512) X(75, 5, 15, 384) X(81, 9, 9, 768) X(100, 10, 10, 0)                       \
+  X(16, 4, 4, 64) X(25, 5, 5, 192) X(32, 4, 8, 128) X(36, 6, 6, 256) X(48, 4, 12, 192)         \
+  X(64, 8, 8, 512) X(75, 5, 15, 448) X(81, 9, 9, 768) X(100, 10, 10, 0)                       \
   X(128, 8, 16, 0) X(144, 12, 12, 0)
 
 template <class P>
@@ -609,9 +708,95 @@ static int run_band(const FpArgs& a, int groups, hipStream_t st) {
   return OI_OK;
 }
 
+// Large coarse grids of the two-pass coherent sum.
+#define OI_TWO_PASS_PLANS(X) X(256, 16, 16) X(512, 16, 32)
+
+template <class PL>
+static size_t two_pass_rows_lds(int wc) {
+  return ((size_t)wc * (groups_per_block<PL>() + 1) + (size_t)scratch_groups<PL>() * PL::LDS) *
+         sizeof(float2);
+}
+
+template <class PL>
+static int run_two_pass(const FpArgs& a, const FpCoh& c, float2* B, int generic_layout,
+                        hipStream_t st) {
+  constexpr int G = groups_per_block<PL>();
+  const size_t lds1 = rows_lds<PL>();
+  int rc = allow(k_fp_bcols<PL>, lds1);
+  if (rc) return rc;
+  dim3 g1((a.wc + G - 1) / G, c.nkern * c.nchan, a.batch);
+  hipLaunchKernelGGL(k_fp_bcols<PL>, g1, dim3(kBlock), lds1, st, c.spec, c.kern, c.nkern, c.nchan,
+                     a.k0r, a.wr, a.wc, a.tw_l, B);
+  OI_LAUNCH_CHECK("fp_bcols");
+  const size_t lds2 = two_pass_rows_lds<PL>(a.wc);
+  rc = allow(k_fp_brows<PL>, lds2);
+  if (rc) return rc;
+  dim3 g2((PL::N + G - 1) / G, a.batch);
+  hipLaunchKernelGGL(k_fp_brows<PL>, g2, dim3(kBlock), lds2, st, B, c.nkern * c.nchan, a.k0c, a.wc,
+                     a.qc, generic_layout, a.tw_l, c.R);
+  OI_LAUNCH_CHECK("fp_brows");
+  return OI_OK;
+}
+
 }  // namespace fp
 
 // ------------------------------------------------------------ dispatch
+bool fp_two_pass_size_ok(int l) {
+#define X(NN, A, B) if (l == NN) return true;
+  OI_TWO_PASS_PLANS(X)
+#undef X
+  return false;
+}
+
+int fp_two_pass_choose(int need) {
+  int best = 0;
+#define X(NN, A, B) if (NN >= need && (best == 0 || NN < best)) best = NN;
+  OI_TWO_PASS_PLANS(X)
+#undef X
+  return best;
+}
+
+size_t fp_two_pass_lds(int L, int wc) {
+#define X(NN, A, B) if (L == NN) return fp::two_pass_rows_lds<reg::GFFT<NN, A, B>>(wc);
+  OI_TWO_PASS_PLANS(X)
+#undef X
+  return (size_t)-1;
+}
+
+int fp_two_pass(const FpArgs& a, const FpCoh& c, float2* B, int generic_layout, hipStream_t st) {
+  int rc = OI_ERR_UNSUPPORTED;
+#define X(NN, A, B_) if (a.L == NN) rc = fp::run_two_pass<reg::GFFT<NN, A, B_>>(a, c, B, generic_layout, st);
+  OI_TWO_PASS_PLANS(X)
+#undef X
+  if (rc == OI_ERR_UNSUPPORTED) return fail(rc, "two-pass path: unsupported coarse size");
+  return rc;
+}
+
+// Band extraction + column and row transforms to the image (stages 4, 5) for
+// partial sums R[img][groups][q][t] on an L x L coarse grid.
+int fp_interp(const FpArgs& a, int groups, hipStream_t st) {
+  int rc = OI_ERR_UNSUPPORTED;
+#define X(NN, A, B, NT) if (a.L == NN) rc = fp::run_band<reg::GFFT<NN, A, B>>(a, groups, st);
+  OI_COARSE_PLANS(X)
+#undef X
+#define X(NN, A, B) if (a.L == NN) rc = fp::run_band<reg::GFFT<NN, A, B>>(a, groups, st);
+  OI_TWO_PASS_PLANS(X)
+#undef X
+  if (rc) return rc == OI_ERR_UNSUPPORTED ? fail(rc, "fast path: unsupported coarse size") : rc;
+  rc = OI_ERR_UNSUPPORTED;
+#define X(NN, A, B) if (a.ny == NN) rc = fp::run_ucols<reg::GFFT<NN, A, B>>(a, st);
+  OI_IMAGE_PLANS(X)
+#undef X
+  if (rc) return rc;
+  if (a.mark) a.mark(a.mark_ctx, 4);
+  rc = OI_ERR_UNSUPPORTED;
+#define X(NN, A, B) if (a.nx == NN) rc = fp::run_urows<reg::GFFT<NN, A, B>>(a, st);
+  OI_IMAGE_PLANS(X)
+#undef X
+  if (rc) return rc;
+  if (a.mark) a.mark(a.mark_ctx, 5);
+  return OI_OK;
+}
 bool fp_image_size_ok(int n) {
 #define X(NN, A, B) if (n == NN) return true;
   OI_IMAGE_PLANS(X)
@@ -679,24 +864,7 @@ int fp_image(const FpArgs& a, const FpCoh& c, hipStream_t st) {
 #undef X
   if (rc) return rc == OI_ERR_UNSUPPORTED ? fail(rc, "fast path: unsupported coarse size") : rc;
   if (a.mark) a.mark(a.mark_ctx, 3);
-  rc = OI_ERR_UNSUPPORTED;
-#define X(NN, A, B, NT) if (a.L == NN) rc = fp::run_band<reg::GFFT<NN, A, B>>(a, c.groups, st);
-  OI_COARSE_PLANS(X)
-#undef X
-  if (rc) return rc;
-  rc = OI_ERR_UNSUPPORTED;
-#define X(NN, A, B) if (a.ny == NN) rc = fp::run_ucols<reg::GFFT<NN, A, B>>(a, st);
-  OI_IMAGE_PLANS(X)
-#undef X
-  if (rc) return rc;
-  if (a.mark) a.mark(a.mark_ctx, 4);
-  rc = OI_ERR_UNSUPPORTED;
-#define X(NN, A, B) if (a.nx == NN) rc = fp::run_urows<reg::GFFT<NN, A, B>>(a, st);
-  OI_IMAGE_PLANS(X)
-#undef X
-  if (rc) return rc;
-  if (a.mark) a.mark(a.mark_ctx, 5);
-  return OI_OK;
+  return fp_interp(a, c.groups, st);
 }
 
 }  // namespace oi

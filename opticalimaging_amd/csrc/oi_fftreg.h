@@ -167,6 +167,14 @@ struct DFT {
   }
 };
 
+// Orders this wavefront's LDS accesses before and after the call (no
+// workgroup barrier): fences at wavefront scope around a scheduling barrier.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Padded LDS index: one spare slot every 32 entries breaks power-of-two strides.
 __device__ __forceinline__ int pad(int x) { return x + (x >> 5); }
 __host__ __device__ constexpr int padded(int n) { return n + (n >> 5) + 1; }
@@ -180,6 +188,12 @@ struct GFFT {
   static constexpr int T = N / V;
   static constexpr int U1 = V / R1;                 // stage-1 butterflies per thread
   static constexpr int U2 = R2 == 1 ? 1 : V / R2;   // stage-2 butterflies per thread
+  // Exchange index: sizes that are multiples of 32 take one spare slot every
+  // 32 entries (power-of-two strides would meet in one bank); other sizes use
+  // the plain index, which keeps every LDS address a per-thread base plus an
+  // immediate offset.
+  static constexpr bool PAD = (N % 32) == 0;
+  __device__ __forceinline__ static int xi(int i) { return PAD ? pad(i) : i; }
   static constexpr int LDS = R2 == 1 ? 0 : padded(N);  // scratch (float2) per group
   static_assert(R2 == 1 ? R1 == N : (R1 * R2 == N && (V % R1) == 0 && (V % R2) == 0),
                 "GFFT: N = R1 R2 with the smaller radix dividing the larger");
@@ -205,7 +219,7 @@ struct GFFT {
   // every thread of the block must call (barriers inside) when R2 > 1.
   template <bool INV>
   __device__ __forceinline__ static void run(float2 (&v)[V], float2* lds, int t, const Tw& tw) {
-    run_impl<INV>(v, [lds](int i) -> float2& { return lds[pad(i)]; }, t, tw, true);
+    run_impl<INV>(v, [lds](int i) -> float2& { return lds[xi(i)]; }, t, tw, true);
   }
 
   // The same transform exchanging through base[i * stride], i in [0, N) (no
@@ -220,7 +234,7 @@ struct GFFT {
   template <bool INV>
   __device__ __forceinline__ static void run_live(float2 (&v)[V], float2* lds, int t, const Tw& tw,
                                                   bool live) {
-    run_impl<INV>(v, [lds](int i) -> float2& { return lds[pad(i)]; }, t, tw, live);
+    run_impl<INV>(v, [lds](int i) -> float2& { return lds[xi(i)]; }, t, tw, live);
   }
 
   // Twiddles read from a table at use (table[m] = exp(-2 pi i m / N), in LDS
@@ -245,7 +259,7 @@ struct GFFT {
   template <bool INV>
   __device__ __forceinline__ static void run_live_tab(float2 (&v)[V], float2* lds, int t,
                                                       const float2* table, bool live) {
-    run_gen<INV>(v, [lds](int i) -> float2& { return lds[pad(i)]; }, t, TabTw{table}, live);
+    run_gen<INV>(v, [lds](int i) -> float2& { return lds[xi(i)]; }, t, TabTw{table}, live);
   }
 
   template <bool INV, class Addr>
@@ -254,7 +268,22 @@ struct GFFT {
     run_gen<INV>(v, at, t, RegTw{tw}, live);
   }
 
-  template <bool INV, class Addr, class TW>
+  // Exchange within one wavefront (the group's lanes and its scratch belong to
+  // a single wave): LDS operations of a wave complete in order, so a compiler
+  // barrier with wavefront-scope fences replaces the workgroup barrier.
+  template <bool INV>
+  __device__ __forceinline__ static void run_strided_wave(float2 (&v)[V], float2* base, int stride,
+                                                          int t, const Tw& tw, bool live) {
+    run_gen<INV, true>(v, [base, stride](int i) -> float2& { return base[i * stride]; }, t,
+                       RegTw{tw}, live);
+  }
+  template <bool INV>
+  __device__ __forceinline__ static void run_live_wave(float2 (&v)[V], float2* lds, int t,
+                                                       const Tw& tw, bool live) {
+    run_gen<INV, true>(v, [lds](int i) -> float2& { return lds[xi(i)]; }, t, RegTw{tw}, live);
+  }
+
+  template <bool INV, bool WAVE = false, class Addr, class TW>
   __device__ __forceinline__ static void run_gen(float2 (&v)[V], Addr at, int t, const TW& tw,
                                                  bool live) {
     if constexpr (R2 == 1) {
@@ -272,7 +301,7 @@ struct GFFT {
           for (int j = 0; j < R1; ++j) at(i * R1 + j) = b[j];
         }
       }
-      __syncthreads();
+      sync<WAVE>();
 #pragma unroll
       for (int u = 0; u < U2; ++u) {
         const int i = t + T * u;
@@ -288,6 +317,15 @@ struct GFFT {
 #pragma unroll
         for (int j = 0; j < R2; ++j) v[u + j * U2] = b[j];
       }
+      sync<WAVE>();
+    }
+  }
+
+  template <bool WAVE>
+  __device__ __forceinline__ static void sync() {
+    if constexpr (WAVE) {
+      wave_sync();
+    } else {
       __syncthreads();
     }
   }

@@ -94,3 +94,28 @@ def test_fast_abbe_1024_config3_optics(gpu):
     err = rel_err(img.Intensity.numpy(), ref)
     print("fast Abbe 1024 err", err)
     assert err < 1e-5
+
+
+@pytest.mark.parametrize("n", [1024, 2048])
+def test_two_pass_large_coarse_grid(gpu, n):
+    """Kernels of the NA 1.35 quasar optics on 4 nm pixels: windows of 109 / 217
+    orders need coarse grids of 256 / 512 (two-pass coherent sum); 2048^2 also
+    takes the generic spectrum and interpolation kernels."""
+    from opticalimaging_amd import engine, workloads
+    from opticalimaging_amd.DecomposeTCC_SOCS import socs_kernel_set
+    src, proj, num = workloads.ilt_optics()
+    period = 4.0 * n
+    ks = socs_kernel_set(src, [period, period], proj, 0.0, num, [n, n])
+    plan = engine.plan_for(ks, 1, 2)
+    info = plan.info()
+    assert info["path"] == 2 and info["lr"] == (256 if n == 1024 else 512), info
+    masks = np.stack([workloads.serving_mask(s, n) for s in (5, 6)])
+    img = engine.socs_image(torch.as_tensor(masks).cuda(), ks, plan=plan).cpu().numpy()
+    dense = ks.dense().cpu().numpy().astype(np.complex128)
+    for b in range(2):
+        f3 = np.zeros((n, n, 3))
+        f3[..., 0] = masks[b]
+        ref = O.socs_image(f3, dense, period, period, proj.NA, src.Wavelength)
+        err = rel_err(img[b], ref)
+        print(f"two-pass {n}^2 coarse {info['lr']} window {ks.wr} img {b}: {err:.2e}")
+        assert err < 1e-5

@@ -270,20 +270,35 @@ __global__ __launch_bounds__(kBlock) void k_fp_coherent(FpCoh A) {
 // every window column in one pass.  Groups of T_L lanes never straddle a
 // wavefront (GW = 64 / T_L groups per wave, NT / 64 waves, NT/64 * GW >= L), so
 // each transform exchanges with wave-level ordering only.  All transforms work
-// in place in one LDS tile [L][S] (S = padded L): the column stage of window
-// column kc in tile column (k0c + kc) mod L, the row stage of coarse row a in
-// tile row a.  Two workgroup barriers per kernel: after the column stage (rows
-// read every column) and after the row stage (the next column stage writes
-// across rows).  Requires wr, wc <= L (no fold collisions).
+// in place in one LDS tile [L][S]: the column stage of window column kc in tile
+// column (k0c + kc) mod L, the row stage of coarse row a in tile row a.  Two
+// workgroup barriers per kernel: after the column stage (rows read every
+// column) and after the row stage (the next column stage writes across rows).
+// The image's spectrum window sits in LDS per channel; each thread's kernel
+// values are loaded one kernel ahead.  Requires wr, wc <= L.
+template <int L>
+constexpr int coh_stride() {
+  // row stride of the tile in float2; 89 for L = 75 spreads the in-place
+  // transform accesses over the 64-bit banks (fewest conflicts in a model of
+  // the access pattern), padded L otherwise
+  return L == 75 ? 89 : reg::padded(L);
+}
+
+template <class PL>
+size_t coh_lds_bytes(int wr, int wc) {
+  return ((size_t)PL::N * coh_stride<PL::N>() + (size_t)wr * wc) * sizeof(float2);
+}
+
 template <class PL, int NT>
 __global__ __launch_bounds__(NT) void k_fp_coh(FpCoh A) {
   constexpr int L = PL::N, TL = PL::T, VL = PL::V;
-  constexpr int S = reg::padded(L);
+  constexpr int S = coh_stride<L>();
   constexpr int GW = 64 / TL;
   static_assert(GW >= 1 && (NT / 64) * GW >= L, "one group per coarse row");
   static_assert(VL <= 32, "row mask holds VL bits");
-  extern __shared__ float2 lds[];  // tile [L][S]
+  extern __shared__ float2 lds[];  // tile [L][S], then the spectrum window [wr][wc]
   const int wr = A.wr, wc = A.wc;
+  float2* specL = lds + L * S;
   const int img = blockIdx.x, grp = blockIdx.y;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int gw = lane / TL, t = lane - gw * TL;
@@ -294,12 +309,20 @@ __global__ __launch_bounds__(NT) void k_fp_coh(FpCoh A) {
   typename PL::Tw tw;
   PL::load_tw(tw, t, A.tw_l);
   const int kr0 = pmod(t - A.k0r, L);     // TL m < L: kr = kr0 + TL m, less L once
+  unsigned cmask = 0;                     // column-stage values inside the window
+#pragma unroll
+  for (int m = 0; m < VL; ++m) {
+    int kr = kr0 + TL * m;
+    kr = kr >= L ? kr - L : kr;
+    if (live_c && kr < wr) cmask |= 1u << m;
+  }
   const int xcol = live_c ? pmod(A.k0c + g, L) : 0;
   unsigned rmask = 0;
 #pragma unroll
   for (int m = 0; m < VL; ++m)
     if (live_r && pmod(t + TL * m - A.k0c, L) < wc) rmask |= 1u << m;
   float2* row = lds + (live_r ? g : 0) * S;
+  const int gc = live_c ? g : 0;
 
   float acc[VL];
 #pragma unroll
@@ -307,14 +330,36 @@ __global__ __launch_bounds__(NT) void k_fp_coh(FpCoh A) {
 
   for (int c = 0; c < A.nchan; ++c) {
     const float2* sp = A.spec + ((size_t)img * A.nchan + c) * wr * wc;
+    __syncthreads();
+    for (int e = threadIdx.x; e < wr * wc; e += NT) specL[e] = sp[e];
+    // this thread's kernel values of the first kernel
+    float2 nxt[VL];
+    {
+      const float2* kp = A.kern + (size_t)grp * wr * wc;
+#pragma unroll
+      for (int m = 0; m < VL; ++m) {
+        int kr = kr0 + TL * m;
+        kr = kr >= L ? kr - L : kr;
+        nxt[m] = (grp < A.nkern && ((cmask >> m) & 1u)) ? kp[kr * wc + gc] : make_float2(0.f, 0.f);
+      }
+    }
+    __syncthreads();
     for (int kk = grp; kk < A.nkern; kk += A.groups) {
-      const float2* kp = A.kern + (size_t)kk * wr * wc;
       float2 v[VL];
 #pragma unroll
       for (int m = 0; m < VL; ++m) {
         int kr = kr0 + TL * m;
         kr = kr >= L ? kr - L : kr;
-        v[m] = (live_c && kr < wr) ? reg::mul(kp[kr * wc + g], sp[kr * wc + g]) : make_float2(0.f, 0.f);
+        v[m] = ((cmask >> m) & 1u) ? reg::mul(nxt[m], specL[kr * wc + gc]) : make_float2(0.f, 0.f);
+      }
+      // next kernel's values, in flight during this kernel's transforms
+      const int kn = kk + A.groups;
+      const float2* kp = A.kern + (size_t)(kn < A.nkern ? kn : kk) * wr * wc;
+#pragma unroll
+      for (int m = 0; m < VL; ++m) {
+        int kr = kr0 + TL * m;
+        kr = kr >= L ? kr - L : kr;
+        nxt[m] = ((cmask >> m) & 1u) ? kp[kr * wc + gc] : make_float2(0.f, 0.f);
       }
       PL::template run_strided_wave<false>(v, lds + xcol, S, t, tw, live_c);
       if (live_c) {
@@ -668,7 +713,7 @@ static bool one_pass(const FpArgs& a) {
 
 template <class PL, int NT>
 static size_t coherent_lds(const FpArgs& a) {
-  if (one_pass<PL, NT>(a)) return (size_t)PL::N * reg::padded(PL::N) * sizeof(float2);
+  if (one_pass<PL, NT>(a)) return coh_lds_bytes<PL>(a.wr, a.wc);
   const size_t scr = (size_t)scratch_groups<PL>() * PL::LDS;
   return ((size_t)a.wr * a.wc + (size_t)PL::N * a.wc + scr) * sizeof(float2);
 }

@@ -20,6 +20,8 @@ Also reported, outside the timed region:
   * serving: a Poisson stream of 256^2..2048^2 clips, continuous batching
     under an 8 GB HBM budget, p50/p99 latency and images/s;
   * ILT: 50 gradient steps of ||I(sigmoid(theta)) - T||^2 at 1024^2 x 16;
+  * the scalar 128^2 and vector 512^2 (5 foci) Abbe configurations through
+    CalculateAbbeImage;
   * the float64 CPU oracle timed on a bounded sample (cpu_baseline).
 """
 from __future__ import annotations
@@ -224,6 +226,32 @@ def serving_leg(count, rate):
             "kernel_build_s": round(t_ks, 2), "mask_generation_s": round(t_gen, 2)}
 
 
+def abbe_configs_leg():
+    """The two Abbe configurations through the drop-in API (CalculateAbbeImage),
+    ScatterField resident on the device: scalar 128^2 (annular, 120 source
+    points, 1 focus) and vector 512^2 (NA 1.35 quasar, 196 points x 3
+    polarisation components, 37 Zernikes, 5 foci)."""
+    from opticalimaging_amd import CalculateAbbeImage, Scatter, workloads
+    out = {}
+    cases = (("scalar_128", workloads.scalar_abbe_128()), ("vector_512_5foci", workloads.vector_abbe_512()))
+    for name, (mask, src, proj, num, period) in cases:
+        sc = Scatter()
+        sc.Period_X = sc.Period_Y = period
+        sc.ScatterField = workloads.field_from_mask(mask).cuda()
+        CalculateAbbeImage(src, sc, proj, num)            # warm-up (plans, twiddles)
+        torch.cuda.synchronize()
+        reps = 5
+        t = time.perf_counter()
+        for _ in range(reps):
+            d, _ = CalculateAbbeImage(src, sc, proj, num)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t) / reps
+        nf = int(d.Intensity.shape[0])
+        out[name] = {"image": list(mask.shape), "foci": nf, "ms_per_call": round(dt * 1e3, 3),
+                     "focus_images_per_s": round(nf / dt, 1)}
+    return out
+
+
 def ilt_leg(steps):
     """||I(sigmoid(theta)) - T||^2 descent at 1024^2 x 16 with the adjoint kernel."""
     from opticalimaging_amd import ILT, workloads
@@ -414,6 +442,10 @@ def main():
         if args.serving_requests > 0:
             line["serving"] = serving_leg(args.serving_requests, args.serving_rate)
             log(f"serving: {json.dumps(line['serving'])}")
+
+        # ---------------- the Abbe configurations through the drop-in API
+        line["abbe_configs"] = abbe_configs_leg()
+        log(f"abbe configs: {json.dumps(line['abbe_configs'])}")
 
         # ---------------- ILT: 50 gradient steps at 1024^2 x 16
         if args.ilt_steps > 0:

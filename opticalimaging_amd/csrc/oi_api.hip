@@ -210,8 +210,12 @@ extern "C" int oi_plan_create(oi_geometry* geo, oi_plan** out) {
   // Kernel groups: fixed per plan so an image's summation order never depends
   // on the batch it is computed in.  0 = enough groups for ~512 workgroups.
   if (g.max_groups <= 0) {
+    // about one workgroup per CU for the compile-time-size kernels (one
+    // large block per CU; more partial sums cost more than they hide), about
+    // 512 workgroups for the generic kernels
     const int phases = lr / g.rows;
-    int gr = (512 + phases * g.max_batch - 1) / (phases * g.max_batch);
+    const int target = g.path == 1 ? 256 : 512;
+    int gr = (target + phases * g.max_batch - 1) / (phases * g.max_batch);
     g.max_groups = gr < 1 ? 1 : (gr > 64 ? 64 : gr);
   }
 

@@ -37,6 +37,26 @@ constexpr int scratch_groups() {
 }
 
 // ------------------------------------------------------------------ rows
+// Real masks, two rows per complex transform.  A block walks kRowIter sets of
+// 2G rows, issuing the next set's loads before transforming the current one
+// (the exchange scratch limits a CU to two blocks, so the loads of one set
+// overlap the other work of the block instead of waiting on other blocks).
+constexpr int kRowIter = 1;
+// Transform groups that tile wavefronts exactly exchange with wave-level ordering.
+constexpr bool kRowWave = false;
+
+template <class P>
+__device__ __forceinline__ void load_row_pair(float2 (&v)[P::V], const float* __restrict__ src,
+                                              int r0, int ny, int t, bool live) {
+#pragma unroll
+  for (int m = 0; m < P::V; ++m) {
+    const int x = t + P::T * m;
+    const float a = (live && r0 < ny) ? src[(size_t)r0 * P::N + x] : 0.f;
+    const float b = (live && r0 + 1 < ny) ? src[(size_t)(r0 + 1) * P::N + x] : 0.f;
+    v[m] = make_float2(a, b);
+  }
+}
+
 template <class P>
 __global__ __launch_bounds__(kBlock) void k_fp_rows_real(const float* __restrict__ mask, int ny,
                                                          int k0c, int wc,
@@ -49,32 +69,40 @@ __global__ __launch_bounds__(kBlock) void k_fp_rows_real(const float* __restrict
   const int gg = live ? g : 0;
   float2* scr = lds + g * P::LDS;
   const int img = blockIdx.y;
-  const int r0 = 2 * (blockIdx.x * G + gg), r1 = r0 + 1;
-  typename P::Tw tw;
-  P::load_tw(tw, t, twt);
+  const int sets = (ny + 2 * G - 1) / (2 * G);
+  const int s0 = blockIdx.x * kRowIter;
   const float* src = mask + (size_t)img * ny * N;
-  float2 v[V];
+  float2 nxt[V];
+  load_row_pair<P>(nxt, src, 2 * (s0 * G + gg), ny, t, live);
+  for (int i = 0; i < kRowIter; ++i) {
+    const int set = s0 + i;
+    if (set >= sets) break;                      // uniform over the block
+    float2 v[V];
 #pragma unroll
-  for (int m = 0; m < V; ++m) {
-    const int x = t + TT * m;
-    const float a = (live && r0 < ny) ? src[(size_t)r0 * N + x] : 0.f;
-    const float b = (live && r1 < ny) ? src[(size_t)r1 * N + x] : 0.f;
-    v[m] = make_float2(a, b);
-  }
-  P::template run<true>(v, scr, t, tw);
-  if (live) {
+    for (int m = 0; m < V; ++m) v[m] = nxt[m];
+    if (i + 1 < kRowIter && set + 1 < sets) load_row_pair<P>(nxt, src, 2 * ((set + 1) * G + gg), ny, t, live);
+    const int r0 = 2 * (set * G + gg), r1 = r0 + 1;
+    if constexpr (kRowWave && 64 % TT == 0) {
+      P::template run_tab_wave<true>(v, scr, t, twt, true);
+    } else {
+      P::template run_live_tab<true>(v, scr, t, twt, true);
+    }
+    if (live) {
 #pragma unroll
-    for (int m = 0; m < V; ++m) scr[pad(t + TT * m)] = v[m];
-  }
-  __syncthreads();
-  if (!live || r0 >= ny) return;
-  float2* out0 = T + ((size_t)img * ny + r0) * wc;
-  for (int j = t; j < wc; j += TT) {
-    const int k = pmod(k0c + j, N);
-    const int km = k == 0 ? 0 : N - k;
-    const float2 zk = scr[pad(k)], zm = scr[pad(km)];
-    out0[j] = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));            // (Z[k] + conj Z[-k]) / 2
-    if (r1 < ny) out0[wc + j] = make_float2(0.5f * (zk.y + zm.y), -0.5f * (zk.x - zm.x));  // / 2i
+      for (int m = 0; m < V; ++m) scr[pad(t + TT * m)] = v[m];
+    }
+    if constexpr (kRowWave && 64 % TT == 0) reg::wave_sync(); else __syncthreads();
+    if (live && r0 < ny) {
+      float2* out0 = T + ((size_t)img * ny + r0) * wc;
+      for (int j = t; j < wc; j += TT) {
+        const int k = pmod(k0c + j, N);
+        const int km = k == 0 ? 0 : N - k;
+        const float2 zk = scr[pad(k)], zm = scr[pad(km)];
+        out0[j] = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));            // (Z[k] + conj Z[-k]) / 2
+        if (r1 < ny) out0[wc + j] = make_float2(0.5f * (zk.y + zm.y), -0.5f * (zk.x - zm.x));  // / 2i
+      }
+    }
+    if constexpr (kRowWave && 64 % TT == 0) reg::wave_sync(); else __syncthreads();  // scratch reused
   }
 }
 
@@ -93,15 +121,13 @@ __global__ __launch_bounds__(kBlock) void k_fp_rows_c64(const float2* __restrict
   const int img = bc / nchan, ci = bc - img * nchan;
   const int chan = ch.c[ci];
   const int r = blockIdx.x * G + gg;
-  typename P::Tw tw;
-  P::load_tw(tw, t, twt);
   float2 v[V];
 #pragma unroll
   for (int m = 0; m < V; ++m) {
     const int x = t + TT * m;
     v[m] = (live && r < ny) ? field[(((size_t)img * ny + r) * N + x) * cin + chan] : make_float2(0.f, 0.f);
   }
-  P::template run<true>(v, scr, t, tw);
+  P::template run_live_tab<true>(v, scr, t, twt, true);
   if (live) {
 #pragma unroll
     for (int m = 0; m < V; ++m) scr[pad(t + TT * m)] = v[m];
@@ -126,13 +152,11 @@ __global__ __launch_bounds__(kBlock) void k_fp_cols(const float2* __restrict__ T
   const int bc = blockIdx.y;
   const int j = blockIdx.x * G + gg;
   const bool col = live && j < wc;
-  typename P::Tw tw;
-  P::load_tw(tw, t, twt);
   const float2* src = T + (size_t)bc * N * wc;
   float2 v[V];
 #pragma unroll
   for (int m = 0; m < V; ++m) v[m] = col ? src[(size_t)(t + TT * m) * wc + j] : make_float2(0.f, 0.f);
-  P::template run<true>(v, lds + g * P::LDS, t, tw);
+  P::template run_live_tab<true>(v, lds + g * P::LDS, t, twt, true);
   if (!col) return;
   float2* dst = spec + (size_t)bc * wr * wc;
 #pragma unroll
@@ -509,8 +533,6 @@ __global__ __launch_bounds__(kBandBlock) void k_fp_band(const float2* __restrict
   const int q = blockIdx.x * G + gg;
   const int nq = qc + 1;
   const bool on = live && q < nq;
-  typename PL::Tw tw;
-  PL::load_tw(tw, t, twt);
   float2 v[VL];
 #pragma unroll
   for (int m = 0; m < VL; ++m) {
@@ -526,7 +548,7 @@ __global__ __launch_bounds__(kBandBlock) void k_fp_band(const float2* __restrict
     }
     v[m] = sum;
   }
-  PL::template run<true>(v, lds + g * PL::LDS, t, tw);
+  PL::template run_live_tab<true>(v, lds + g * PL::LDS, t, twt, true);
   if (!on) return;
   const int nb = 2 * qrm + 1;
   float2* dst = Ihat + ((size_t)img * nq + q) * nb;
@@ -552,8 +574,6 @@ __global__ __launch_bounds__(kBlock) void k_fp_ucols(const float2* __restrict__ 
   const int q = blockIdx.x * G + gg;
   const int nq = qc + 1;
   const bool on = live && q < nq;
-  typename P::Tw tw;
-  P::load_tw(tw, t, twt);
   const float2* src = Ihat + ((size_t)img * nq + (on ? q : 0)) * (2 * qrm + 1);
   float2 v[V];
 #pragma unroll
@@ -562,7 +582,7 @@ __global__ __launch_bounds__(kBlock) void k_fp_ucols(const float2* __restrict__ 
     const int qr = y <= qrm ? y : y - N;
     v[m] = (on && qr >= -qrm && qr <= qrm) ? src[qr + qrm] : make_float2(0.f, 0.f);
   }
-  P::template run<false>(v, lds + g * P::LDS, t, tw);
+  P::template run_live_tab<false>(v, lds + g * P::LDS, t, twt, true);
   if (!on) return;
   float2* dst = U + ((size_t)img * nq + q) * N;
 #pragma unroll
@@ -575,10 +595,12 @@ __global__ __launch_bounds__(kBlock) void k_fp_urows(const float2* __restrict__ 
                                                      const float2* __restrict__ twt,
                                                      float* __restrict__ out) {
   constexpr int N = P::N, TT = P::T, V = P::V, G = groups_per_block<P>();
+  constexpr int TS = 2 * G + 1;               // tile row stride (odd: lanes reading
+                                              // consecutive q hit different banks)
   extern __shared__ float2 lds[];
   const int nq = qc + 1;
-  float2* tile = lds;                         // [nq][2G]
-  float2* scr = lds + nq * 2 * G;
+  float2* tile = lds;                         // [nq][TS]
+  float2* scr = lds + nq * TS;
   const int g = threadIdx.x / TT, t = threadIdx.x - g * TT;
   const bool live = g < G;
   const int gg = live ? g : 0;
@@ -587,11 +609,9 @@ __global__ __launch_bounds__(kBlock) void k_fp_urows(const float2* __restrict__ 
   for (int e = threadIdx.x; e < nq * 2 * G; e += kBlock) {
     const int q = e / (2 * G), r = e - q * 2 * G;
     const int y = y0 + r;
-    tile[e] = y < ny ? U[((size_t)img * nq + q) * ny + y] : make_float2(0.f, 0.f);
+    tile[q * TS + r] = y < ny ? U[((size_t)img * nq + q) * ny + y] : make_float2(0.f, 0.f);
   }
   __syncthreads();
-  typename P::Tw tw;
-  P::load_tw(tw, t, twt);
   float2 v[V];
 #pragma unroll
   for (int m = 0; m < V; ++m) {
@@ -600,7 +620,7 @@ __global__ __launch_bounds__(kBlock) void k_fp_urows(const float2* __restrict__ 
     if (live && (x <= qc || x >= N - qc)) {
       const bool neg = x > qc;
       const int q = neg ? N - x : x;
-      float2 u0 = tile[q * 2 * G + 2 * gg], u1 = tile[q * 2 * G + 2 * gg + 1];
+      float2 u0 = tile[q * TS + 2 * gg], u1 = tile[q * TS + 2 * gg + 1];
       if (neg) {
         u0.y = -u0.y;
         u1.y = -u1.y;
@@ -609,7 +629,7 @@ __global__ __launch_bounds__(kBlock) void k_fp_urows(const float2* __restrict__ 
     }
     v[m] = z;
   }
-  P::template run<false>(v, scr + g * P::LDS, t, tw);
+  P::template run_live_tab<false>(v, scr + g * P::LDS, t, twt, true);
   if (!live) return;
   const int ya = y0 + 2 * gg, yb = ya + 1;
 #pragma unroll
@@ -651,7 +671,8 @@ static int run_rows(const FpArgs& a, hipStream_t st) {
   if (a.kind == OI_FIELD_REAL) {
     int rc = allow(k_fp_rows_real<P>, lds);
     if (rc) return rc;
-    dim3 grid((a.ny + 2 * G - 1) / (2 * G), a.batch);
+    const int sets = (a.ny + 2 * G - 1) / (2 * G);
+    dim3 grid((sets + kRowIter - 1) / kRowIter, a.batch);
     hipLaunchKernelGGL(k_fp_rows_real<P>, grid, dim3(kBlock), lds, st,
                        static_cast<const float*>(a.field), a.ny, a.k0c, a.wc, a.tw_x, a.T);
   } else {
@@ -695,7 +716,7 @@ static int run_ucols(const FpArgs& a, hipStream_t st) {
 template <class P>
 static int run_urows(const FpArgs& a, hipStream_t st) {
   constexpr int G = groups_per_block<P>();
-  const size_t lds = rows_lds<P>() + (size_t)(a.qc + 1) * 2 * G * sizeof(float2);
+  const size_t lds = rows_lds<P>() + (size_t)(a.qc + 1) * (2 * G + 1) * sizeof(float2);
   int rc = allow(k_fp_urows<P>, lds);
   if (rc) return rc;
   dim3 grid((a.ny + 2 * G - 1) / (2 * G), a.batch);

@@ -10,6 +10,7 @@
 //   k_fp_band<PL>       R columns -> band of the coarse intensity spectrum Ihat
 //   k_fp_ucols<P>       Ihat columns zero-padded to ny -> U[img][q][y]
 //   k_fp_urows<P>       U rows (Hermitian) -> image rows (C2R)
+#include <cmath>
 #include <type_traits>
 
 #include "oi_fastpath.h"
@@ -37,26 +38,8 @@ constexpr int scratch_groups() {
 }
 
 // ------------------------------------------------------------------ rows
-// Real masks, two rows per complex transform.  A block walks kRowIter sets of
-// 2G rows, issuing the next set's loads before transforming the current one
-// (the exchange scratch limits a CU to two blocks, so the loads of one set
-// overlap the other work of the block instead of waiting on other blocks).
-constexpr int kRowIter = 1;
-// Transform groups that tile wavefronts exactly exchange with wave-level ordering.
-constexpr bool kRowWave = false;
-
-template <class P>
-__device__ __forceinline__ void load_row_pair(float2 (&v)[P::V], const float* __restrict__ src,
-                                              int r0, int ny, int t, bool live) {
-#pragma unroll
-  for (int m = 0; m < P::V; ++m) {
-    const int x = t + P::T * m;
-    const float a = (live && r0 < ny) ? src[(size_t)r0 * P::N + x] : 0.f;
-    const float b = (live && r0 + 1 < ny) ? src[(size_t)(r0 + 1) * P::N + x] : 0.f;
-    v[m] = make_float2(a, b);
-  }
-}
-
+// Real masks, two rows per complex transform (real parts row r0, imaginary
+// parts row r0 + 1), outputs separated by Hermitian symmetry.
 template <class P>
 __global__ __launch_bounds__(kBlock) void k_fp_rows_real(const float* __restrict__ mask, int ny,
                                                          int k0c, int wc,
@@ -69,40 +52,30 @@ __global__ __launch_bounds__(kBlock) void k_fp_rows_real(const float* __restrict
   const int gg = live ? g : 0;
   float2* scr = lds + g * P::LDS;
   const int img = blockIdx.y;
-  const int sets = (ny + 2 * G - 1) / (2 * G);
-  const int s0 = blockIdx.x * kRowIter;
+  const int r0 = 2 * (blockIdx.x * G + gg), r1 = r0 + 1;
   const float* src = mask + (size_t)img * ny * N;
-  float2 nxt[V];
-  load_row_pair<P>(nxt, src, 2 * (s0 * G + gg), ny, t, live);
-  for (int i = 0; i < kRowIter; ++i) {
-    const int set = s0 + i;
-    if (set >= sets) break;                      // uniform over the block
-    float2 v[V];
+  float2 v[V];
 #pragma unroll
-    for (int m = 0; m < V; ++m) v[m] = nxt[m];
-    if (i + 1 < kRowIter && set + 1 < sets) load_row_pair<P>(nxt, src, 2 * ((set + 1) * G + gg), ny, t, live);
-    const int r0 = 2 * (set * G + gg), r1 = r0 + 1;
-    if constexpr (kRowWave && 64 % TT == 0) {
-      P::template run_tab_wave<true>(v, scr, t, twt, true);
-    } else {
-      P::template run_live_tab<true>(v, scr, t, twt, true);
-    }
-    if (live) {
+  for (int m = 0; m < V; ++m) {
+    const int x = t + TT * m;
+    const float a = (live && r0 < ny) ? src[(size_t)r0 * N + x] : 0.f;
+    const float b = (live && r1 < ny) ? src[(size_t)r1 * N + x] : 0.f;
+    v[m] = make_float2(a, b);
+  }
+  P::template run_live_tab<true>(v, scr, t, twt, true);
+  if (live) {
 #pragma unroll
-      for (int m = 0; m < V; ++m) scr[pad(t + TT * m)] = v[m];
-    }
-    if constexpr (kRowWave && 64 % TT == 0) reg::wave_sync(); else __syncthreads();
-    if (live && r0 < ny) {
-      float2* out0 = T + ((size_t)img * ny + r0) * wc;
-      for (int j = t; j < wc; j += TT) {
-        const int k = pmod(k0c + j, N);
-        const int km = k == 0 ? 0 : N - k;
-        const float2 zk = scr[pad(k)], zm = scr[pad(km)];
-        out0[j] = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));            // (Z[k] + conj Z[-k]) / 2
-        if (r1 < ny) out0[wc + j] = make_float2(0.5f * (zk.y + zm.y), -0.5f * (zk.x - zm.x));  // / 2i
-      }
-    }
-    if constexpr (kRowWave && 64 % TT == 0) reg::wave_sync(); else __syncthreads();  // scratch reused
+    for (int m = 0; m < V; ++m) scr[pad(t + TT * m)] = v[m];
+  }
+  __syncthreads();
+  if (!live || r0 >= ny) return;
+  float2* out0 = T + ((size_t)img * ny + r0) * wc;
+  for (int j = t; j < wc; j += TT) {
+    const int k = pmod(k0c + j, N);
+    const int km = k == 0 ? 0 : N - k;
+    const float2 zk = scr[pad(k)], zm = scr[pad(km)];
+    out0[j] = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));            // (Z[k] + conj Z[-k]) / 2
+    if (r1 < ny) out0[wc + j] = make_float2(0.5f * (zk.y + zm.y), -0.5f * (zk.x - zm.x));  // / 2i
   }
 }
 
@@ -671,8 +644,7 @@ static int run_rows(const FpArgs& a, hipStream_t st) {
   if (a.kind == OI_FIELD_REAL) {
     int rc = allow(k_fp_rows_real<P>, lds);
     if (rc) return rc;
-    const int sets = (a.ny + 2 * G - 1) / (2 * G);
-    dim3 grid((sets + kRowIter - 1) / kRowIter, a.batch);
+    dim3 grid((a.ny + 2 * G - 1) / (2 * G), a.batch);
     hipLaunchKernelGGL(k_fp_rows_real<P>, grid, dim3(kBlock), lds, st,
                        static_cast<const float*>(a.field), a.ny, a.k0c, a.wc, a.tw_x, a.T);
   } else {

@@ -262,13 +262,6 @@ struct GFFT {
     run_gen<INV>(v, [lds](int i) -> float2& { return lds[xi(i)]; }, t, TabTw{table}, live);
   }
 
-  // Table twiddles and wave-level ordering (the group's lanes lie in one wave).
-  template <bool INV>
-  __device__ __forceinline__ static void run_tab_wave(float2 (&v)[V], float2* lds, int t,
-                                                      const float2* table, bool live) {
-    run_gen<INV, true>(v, [lds](int i) -> float2& { return lds[xi(i)]; }, t, TabTw{table}, live);
-  }
-
   template <bool INV, class Addr>
   __device__ __forceinline__ static void run_impl(float2 (&v)[V], Addr at, int t, const Tw& tw,
                                                   bool live) {

@@ -21,7 +21,7 @@ struct FpArgs {
   const float2* tw_x;  // exp(-2 pi i m / nx)
   const float2* tw_y;  // exp(-2 pi i m / ny)
   const float2* tw_l;  // exp(-2 pi i m / L)
-  float2* T;      // [B][C][ny][wc]
+  float2* T;      // [B][C][wc][ny]
   float2* spec;   // [B][C][wr][wc]
   float2* R;      // [B][G][L][qc+1]
   float2* Ihat;   // [B][qc+1][2qr+1]

@@ -3,7 +3,7 @@
 // kernels of oi_imaging.hip; used when the image sizes and the coarse grid are
 // in the size tables below.
 //
-//   k_fp_rows_real<P>   mask rows -> T[img][r][j] = sum_m h[r][m] e^{+2 pi i (k0c+j) m / nx}
+//   k_fp_rows_real<P>   mask rows -> T[img][j][r] = sum_m h[r][m] e^{+2 pi i (k0c+j) m / nx}
 //   k_fp_rows_c64<P>    same for one channel of a complex channels-last field
 //   k_fp_cols<P>        T columns -> spec[img][c][i][j] (window rows), scaled 1/(ny nx)
 //   k_fp_coherent<PL,PM> spectrum x kernels -> coarse |E|^2 -> row DFT -> R
@@ -68,14 +68,25 @@ __global__ __launch_bounds__(kBlock) void k_fp_rows_real(const float* __restrict
     for (int m = 0; m < V; ++m) scr[pad(t + TT * m)] = v[m];
   }
   __syncthreads();
-  if (!live || r0 >= ny) return;
-  float2* out0 = T + ((size_t)img * ny + r0) * wc;
-  for (int j = t; j < wc; j += TT) {
-    const int k = pmod(k0c + j, N);
-    const int km = k == 0 ? 0 : N - k;
-    const float2 zk = scr[pad(k)], zm = scr[pad(km)];
-    out0[j] = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));            // (Z[k] + conj Z[-k]) / 2
-    if (r1 < ny) out0[wc + j] = make_float2(0.5f * (zk.y + zm.y), -0.5f * (zk.x - zm.x));  // / 2i
+  // outputs of the block's 2G rows go through an LDS tile so that T, which is
+  // column-major per image ([img][j][r]), is written in whole row runs
+  constexpr int TS = 2 * G + 1;
+  float2* tile = lds + scratch_groups<P>() * P::LDS;   // [wc][TS]
+  if (live && r0 < ny) {
+    for (int j = t; j < wc; j += TT) {
+      const int k = pmod(k0c + j, N);
+      const int km = k == 0 ? 0 : N - k;
+      const float2 zk = scr[pad(k)], zm = scr[pad(km)];
+      tile[j * TS + 2 * gg] = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));      // (Z[k] + conj Z[-k]) / 2
+      tile[j * TS + 2 * gg + 1] = make_float2(0.5f * (zk.y + zm.y), -0.5f * (zk.x - zm.x));  // / 2i
+    }
+  }
+  __syncthreads();
+  const int rb = 2 * blockIdx.x * G;
+  float2* Tb = T + (size_t)img * wc * ny;
+  for (int e = threadIdx.x; e < wc * 2 * G; e += kBlock) {
+    const int j = e / (2 * G), rr = e - j * 2 * G;
+    if (rb + rr < ny) Tb[(size_t)j * ny + rb + rr] = tile[j * TS + rr];
   }
 }
 
@@ -106,9 +117,17 @@ __global__ __launch_bounds__(kBlock) void k_fp_rows_c64(const float2* __restrict
     for (int m = 0; m < V; ++m) scr[pad(t + TT * m)] = v[m];
   }
   __syncthreads();
-  if (!live || r >= ny) return;
-  float2* out = T + ((size_t)bc * ny + r) * wc;
-  for (int j = t; j < wc; j += TT) out[j] = scr[pad(pmod(k0c + j, N))];
+  constexpr int TS = G + 1;
+  float2* tile = lds + scratch_groups<P>() * P::LDS;   // [wc][TS]
+  if (live && r < ny)
+    for (int j = t; j < wc; j += TT) tile[j * TS + gg] = scr[pad(pmod(k0c + j, N))];
+  __syncthreads();
+  const int rb = blockIdx.x * G;
+  float2* Tb = T + (size_t)bc * wc * ny;
+  for (int e = threadIdx.x; e < wc * G; e += kBlock) {
+    const int j = e / G, rr = e - j * G;
+    if (rb + rr < ny) Tb[(size_t)j * ny + rb + rr] = tile[j * TS + rr];
+  }
 }
 
 // --------------------------------------------------------------- columns
@@ -125,10 +144,10 @@ __global__ __launch_bounds__(kBlock) void k_fp_cols(const float2* __restrict__ T
   const int bc = blockIdx.y;
   const int j = blockIdx.x * G + gg;
   const bool col = live && j < wc;
-  const float2* src = T + (size_t)bc * N * wc;
+  const float2* src = T + ((size_t)bc * wc + (col ? j : 0)) * N;   // column j, contiguous rows
   float2 v[V];
 #pragma unroll
-  for (int m = 0; m < V; ++m) v[m] = col ? src[(size_t)(t + TT * m) * wc + j] : make_float2(0.f, 0.f);
+  for (int m = 0; m < V; ++m) v[m] = col ? src[t + TT * m] : make_float2(0.f, 0.f);
   P::template run_live_tab<true>(v, lds + g * P::LDS, t, twt, true);
   if (!col) return;
   float2* dst = spec + (size_t)bc * wr * wc;
@@ -640,7 +659,8 @@ static int allow(K kernel, size_t bytes) {
 template <class P>
 static int run_rows(const FpArgs& a, hipStream_t st) {
   constexpr int G = groups_per_block<P>();
-  const size_t lds = rows_lds<P>();
+  // transform scratch + output tile [wc][2G + 1]
+  const size_t lds = rows_lds<P>() + (size_t)a.wc * (2 * G + 1) * sizeof(float2);
   if (a.kind == OI_FIELD_REAL) {
     int rc = allow(k_fp_rows_real<P>, lds);
     if (rc) return rc;

@@ -97,77 +97,144 @@ __device__ __forceinline__ void load_frag(const T* __restrict__ P, int M, int Kp
   }
 }
 
+// Orders this wavefront's LDS accesses (a wavefront's LDS operations complete
+// in order; the fences and the scheduling barrier keep the compiler from
+// moving them).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+constexpr int kMirStride = 17;   // float2 per row of the 16 x 16 transpose buffer
+
 // Writes a 16 x 16 accumulator fragment and, for off-diagonal tiles, its
 // conjugate transpose.  Lane l holds column l&15 and rows 4(l>>4)+i for the
 // fp32-accumulating forms, rows (l>>4)+4i for the fp64 form (INTERLEAVED).
+// Row-major stores of the fragment are 16 consecutive entries per row; the
+// transpose goes through the wave's LDS buffer (16 x 17 float2) so that it is
+// written as 16 consecutive entries per row too.
 template <bool INTERLEAVED, typename A>
 __device__ __forceinline__ void store_frag(float2* __restrict__ tcc, int M, int r0, int c0,
-                                           const A& re, const A& im, double alpha, bool mirror) {
+                                           const A& re, const A& im, double alpha, bool mirror,
+                                           float2* buf) {
   const int lane = threadIdx.x & 63;
-  const int col = c0 + (lane & 15);
+  const int cl = lane & 15;
+  const int col = c0 + cl;
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
-    const int row = r0 + (INTERLEAVED ? (lane >> 4) + 4 * i : (lane >> 4) * 4 + i);
-    if (row < M && col < M) {
-      const float vr = (float)(alpha * (double)re[i]);
-      const float vi = (float)(alpha * (double)im[i]);
-      tcc[(size_t)row * M + col] = make_float2(vr, vi);
-      if (mirror) tcc[(size_t)col * M + row] = make_float2(vr, -vi);
-    }
+    const int rl = INTERLEAVED ? (lane >> 4) + 4 * i : (lane >> 4) * 4 + i;
+    const int row = r0 + rl;
+    const float vr = (float)(alpha * (double)re[i]);
+    const float vi = (float)(alpha * (double)im[i]);
+    if (row < M && col < M) tcc[(size_t)row * M + col] = make_float2(vr, vi);
+    if (mirror) buf[cl * kMirStride + rl] = make_float2(vr, -vi);
+  }
+  if (!mirror) return;   // uniform over the workgroup
+  wave_lds_sync();
+  const int mr = c0 + (lane >> 2);          // row of the transpose = fragment column
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int e = (lane & 3) * 4 + i;       // fragment row
+    if (mr < M && r0 + e < M) tcc[(size_t)mr * M + r0 + e] = buf[(lane >> 2) * kMirStride + e];
+  }
+  wave_lds_sync();                          // the buffer is rewritten by the next fragment
+}
+
+// Split-bf16 contraction: workgroup tile kTileB x kTileB (4 waves in 2 x 2,
+// each (kTileB/2)^2 as FT x FT MFMA tiles; 64 measured faster than 128 at
+// M = 10201: more workgroups per CU overlap one's stores with another's MFMAs).  Panels of a k-step (32 wide) are
+// staged in LDS: A rows (tile I) and X, Z rows (tile J), hi and lo each; rows
+// padded to 40 halves (16-byte aligned, spread over banks).  The next k-step's
+// panels are loaded into registers while this step's MFMAs run.
+constexpr int kTileB = 64;
+constexpr int kPanelK = 32, kPanelRow = 40;
+constexpr int kPanels = 6;                      // Ah Al Xh Xl Zh Zl
+constexpr int kPanelChunks = kPanels * kTileB * 4 / 256;   // 16-byte chunks per thread
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ void tcc_panel_load(const unsigned short* __restrict__ planes, size_t plane,
+                                               int M, int Kp, int rI, int rJ, int k0,
+                                               u32x4 (&reg)[kPanelChunks]) {
+#pragma unroll
+  for (int q = 0; q < kPanelChunks; ++q) {
+    const int c = threadIdx.x + 256 * q;
+    const int panel = c / (kTileB * 4), rem = c - panel * kTileB * 4, row = rem >> 2, ch = rem & 3;
+    // panel order: A hi (X hi rows of I), A lo, X hi (rows of J), X lo, Z hi, Z lo
+    const int pl = panel < 2 ? panel : panel - 2;
+    const int r = (panel < 2 ? rI : rJ) + row;
+    reg[q] = r < M ? *reinterpret_cast<const u32x4*>(planes + pl * plane + (size_t)r * Kp + k0 + ch * 8)
+                   : u32x4{0u, 0u, 0u, 0u};
+  }
+}
+
+__device__ __forceinline__ void tcc_panel_store(unsigned short* lds, const u32x4 (&reg)[kPanelChunks]) {
+#pragma unroll
+  for (int q = 0; q < kPanelChunks; ++q) {
+    const int c = threadIdx.x + 256 * q;
+    const int panel = c / (kTileB * 4), rem = c - panel * kTileB * 4, row = rem >> 2, ch = rem & 3;
+    *reinterpret_cast<u32x4*>(lds + (panel * kTileB + row) * kPanelRow + ch * 8) = reg[q];
   }
 }
 
 __global__ __launch_bounds__(256) void k_tcc_bf16(const unsigned short* __restrict__ planes, int M,
                                                   int Kp, int nt, double alpha,
                                                   float2* __restrict__ tcc) {
+  constexpr int FT = kTileB / 32;               // MFMA tiles per wave and dimension
+  __shared__ unsigned short pan[kPanels * kTileB * kPanelRow];
   const size_t plane = (size_t)M * Kp;
-  const __bf16* Xh = reinterpret_cast<const __bf16*>(planes);
-  const __bf16* Xl = Xh + plane;
-  const __bf16* Zh = Xh + 2 * plane;
-  const __bf16* Zl = Xh + 3 * plane;
   int I, J;
   tile_pair(blockIdx.x, nt, I, J);
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int r0 = I * kTile + (wave >> 1) * 64;
-  const int c0 = J * kTile + (wave & 1) * 64;
+  const int ar = (wave >> 1) * (kTileB / 2), bc = (wave & 1) * (kTileB / 2);
+  const int r0 = I * kTileB + ar, c0 = J * kTileB + bc;
   const int li = lane & 15, lk = (lane >> 4) * 8;
-  f32x4 accR[4][4], accI[4][4];
+  f32x4 accR[FT][FT], accI[FT][FT];
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
+  for (int a = 0; a < FT; ++a)
 #pragma unroll
-    for (int b = 0; b < 4; ++b) {
+    for (int b = 0; b < FT; ++b) {
       accR[a][b] = f32x4{0, 0, 0, 0};
       accI[a][b] = f32x4{0, 0, 0, 0};
     }
-  for (int pass = 0; pass < 3; ++pass) {
-    const __bf16* As = pass == 2 ? Xl : Xh;
-    const __bf16* Br = pass == 1 ? Xl : Xh;
-    const __bf16* Bi = pass == 1 ? Zl : Zh;
-    for (int k0 = 0; k0 < Kp; k0 += 32) {
-      bf16x8 fa[4], fr[4], fi[4];
+  u32x4 nxt[kPanelChunks];
+  tcc_panel_load(planes, plane, M, Kp, I * kTileB, J * kTileB, 0, nxt);
+  for (int k0 = 0; k0 < Kp; k0 += kPanelK) {
+    __syncthreads();                            // previous step's fragment reads are done
+    tcc_panel_store(pan, nxt);
+    __syncthreads();
+    if (k0 + kPanelK < Kp) tcc_panel_load(planes, plane, M, Kp, I * kTileB, J * kTileB, k0 + kPanelK, nxt);
+    bf16x8 ah[FT], al[FT], rh[FT], rl[FT], ih[FT], il[FT];
 #pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        const int ra = r0 + t * 16 + li, cb = c0 + t * 16 + li;
-        const size_t oa = (size_t)ra * Kp + k0 + lk, ob = (size_t)cb * Kp + k0 + lk;
-        fa[t] = ra < M ? *reinterpret_cast<const bf16x8*>(As + oa) : bf16x8{};
-        fr[t] = cb < M ? *reinterpret_cast<const bf16x8*>(Br + ob) : bf16x8{};
-        fi[t] = cb < M ? *reinterpret_cast<const bf16x8*>(Bi + ob) : bf16x8{};
-      }
-#pragma unroll
-      for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int b = 0; b < 4; ++b) {
-          accR[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[a], fr[b], accR[a][b], 0, 0, 0);
-          accI[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[a], fi[b], accI[a][b], 0, 0, 0);
-        }
+    for (int t = 0; t < FT; ++t) {
+      const int ra = ar + t * 16 + li, cb = bc + t * 16 + li;
+      ah[t] = *reinterpret_cast<const bf16x8*>(pan + (0 * kTileB + ra) * kPanelRow + lk);
+      al[t] = *reinterpret_cast<const bf16x8*>(pan + (1 * kTileB + ra) * kPanelRow + lk);
+      rh[t] = *reinterpret_cast<const bf16x8*>(pan + (2 * kTileB + cb) * kPanelRow + lk);
+      rl[t] = *reinterpret_cast<const bf16x8*>(pan + (3 * kTileB + cb) * kPanelRow + lk);
+      ih[t] = *reinterpret_cast<const bf16x8*>(pan + (4 * kTileB + cb) * kPanelRow + lk);
+      il[t] = *reinterpret_cast<const bf16x8*>(pan + (5 * kTileB + cb) * kPanelRow + lk);
     }
+#pragma unroll
+    for (int a = 0; a < FT; ++a)
+#pragma unroll
+      for (int b = 0; b < FT; ++b) {
+        accR[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], rh[b], accR[a][b], 0, 0, 0);
+        accI[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], ih[b], accI[a][b], 0, 0, 0);
+        accR[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], rl[b], accR[a][b], 0, 0, 0);
+        accI[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], il[b], accI[a][b], 0, 0, 0);
+        accR[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], rh[b], accR[a][b], 0, 0, 0);
+        accI[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], ih[b], accI[a][b], 0, 0, 0);
+      }
   }
   const bool mirror = I != J;
+  __shared__ float2 mbuf_all[4 * 16 * kMirStride];
+  float2* mbuf = mbuf_all + (threadIdx.x >> 6) * 16 * kMirStride;
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
+  for (int a = 0; a < FT; ++a)
 #pragma unroll
-    for (int b = 0; b < 4; ++b)
-      store_frag<false>(tcc, M, r0 + a * 16, c0 + b * 16, accR[a][b], accI[a][b], alpha, mirror);
+    for (int b = 0; b < FT; ++b)
+      store_frag<false>(tcc, M, r0 + a * 16, c0 + b * 16, accR[a][b], accI[a][b], alpha, mirror, mbuf);
 }
 
 // fp32 MFMA 16x16x4: lane fragment is one value at k = l>>4; a 16-wide K step
@@ -210,11 +277,13 @@ __global__ __launch_bounds__(256) void k_tcc_f32(const float* __restrict__ plane
         }
   }
   const bool mirror = I != J;
+  __shared__ float2 mbuf_all[4 * 16 * kMirStride];
+  float2* mbuf = mbuf_all + (threadIdx.x >> 6) * 16 * kMirStride;
 #pragma unroll
   for (int a = 0; a < 4; ++a)
 #pragma unroll
     for (int b = 0; b < 4; ++b)
-      store_frag<false>(tcc, M, r0 + a * 16, c0 + b * 16, accR[a][b], accI[a][b], alpha, mirror);
+      store_frag<false>(tcc, M, r0 + a * 16, c0 + b * 16, accR[a][b], accI[a][b], alpha, mirror, mbuf);
 }
 
 // fp64 MFMA 16x16x4 (64 x 32 per wave to bound registers: tile 128 x 64 per block).
@@ -258,11 +327,13 @@ __global__ __launch_bounds__(256) void k_tcc_f64(const double* __restrict__ plan
           }
     }
     const bool mirror = I != J;
+    __shared__ float2 mbuf_all[4 * 16 * kMirStride];
+    float2* mbuf = mbuf_all + (threadIdx.x >> 6) * 16 * kMirStride;
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
       for (int b = 0; b < 2; ++b)
-        store_frag<true>(tcc, M, r0 + a * 16, c0 + b * 16, accR[a][b], accI[a][b], alpha, mirror);
+        store_frag<true>(tcc, M, r0 + a * 16, c0 + b * 16, accR[a][b], accI[a][b], alpha, mirror, mbuf);
   }
 }
 
@@ -362,10 +433,11 @@ extern "C" int oi_tcc_build(const void* G, const float* w, int M, int nsrc, int 
   const int nt = (M + kTile - 1) / kTile;
   const int pairs = nt * (nt + 1) / 2;
   float2* out = static_cast<float2*>(tcc);
-  if (mode == 0)
-    hipLaunchKernelGGL(k_tcc_bf16, dim3(pairs), dim3(256), 0, st,
-                       static_cast<const unsigned short*>(work), M, Kp, nt, alpha, out);
-  else if (mode == 1)
+  if (mode == 0) {
+    const int ntb = (M + kTileB - 1) / kTileB;
+    hipLaunchKernelGGL(k_tcc_bf16, dim3(ntb * (ntb + 1) / 2), dim3(256), 0, st,
+                       static_cast<const unsigned short*>(work), M, Kp, ntb, alpha, out);
+  } else if (mode == 1)
     hipLaunchKernelGGL(k_tcc_f32, dim3(pairs), dim3(256), 0, st, static_cast<const float*>(work), M,
                        Kp, nt, alpha, out);
   else

@@ -204,3 +204,18 @@ def test_ilt_gradient_fd_autograd_and_descent(gpu):
     lo = losses.cpu().numpy()
     print(f"ILT 50 steps: loss {lo[0]:.4e} -> {lo[-1]:.4e}")
     assert np.all(np.isfinite(lo)) and lo[-1] < 0.5 * lo[0]
+
+
+def test_adjoint_rejects_complex_field_plans(gpu):
+    """The adjoint is defined for real masks: a plan carrying complex channels
+    is refused with ValueError (status OI_ERR_INVALID), nothing is launched."""
+    from opticalimaging_amd import engine
+    import opticalimaging_amd._native as nat
+    src, proj, num = annular(), projector(), numerics(order=8)
+    tcc, fg = O.tcc_matrix(src, [1000.0, 1000.0], proj, 0.0, num)
+    kern = O.decompose_tcc_socs(tcc, fg, [64, 64], num)
+    ks = engine.KernelSet.from_dense(torch.as_tensor(kern.astype(np.complex64)))
+    plan = engine.Plan(ks.ny, ks.nx, ks.k0r, ks.wr, ks.k0c, ks.wc, nchan=2, max_batch=1)
+    m = torch.zeros((1, 64, 64), device="cuda")
+    with pytest.raises(ValueError):
+        engine.socs_adjoint(m, m.clone(), ks, plan=plan)

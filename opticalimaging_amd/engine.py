@@ -14,7 +14,7 @@ from __future__ import annotations
 
 import collections
 import ctypes
-from dataclasses import dataclass
+from dataclasses import dataclass, field
 
 import torch
 
@@ -38,6 +38,15 @@ class KernelSet:
     # eigenpair of its TCC: the image band is then the TCC band, 2 floor(2 p),
     # narrower than twice the kernels' support.  None otherwise.
     band: tuple | None = None
+    # kernel set used for real masks (real_mask_kernels), built on first use
+    _real: object = field(default=None, repr=False, compare=False)
+
+    def for_real_masks(self) -> "KernelSet":
+        """The kernel set that images real masks exactly like this one with
+        the fewest kernels (real_mask_kernels; cached)."""
+        if self._real is None:
+            self._real = real_mask_kernels(self)
+        return self._real
 
     @property
     def K(self) -> int:
@@ -88,6 +97,65 @@ class KernelSet:
                                              self.k0c, self.wc, self.ny, self.nx, nat.ptr(out),
                                              nat.stream()))
         return out
+
+
+PAIR_MAX_K = 512        # larger sets are imaged as they are (the 2K x 2K eigenproblem grows)
+
+
+def real_mask_kernels(ks: KernelSet, rel_tol: float = 1e-10) -> KernelSet:
+    """Kernel set with at most ks.K kernels whose SOCS image of every real mask
+    equals that of `ks` (CalculateAerialImage_SOCS.py:3-11).
+
+    For a real mask the spectrum is Hermitian, S(-kappa) = conj S(kappa), so
+    the antilinear map (J phi)(kappa) = conj phi(-kappa) sends a kernel's field
+    to its conjugate, E_{J phi} = conj E_phi.  A kernel with J chi = chi
+    ("J-real") therefore has a real field, and two of them share one complex
+    kernel exactly: |E_{chi_a + i chi_b}|^2 = E_{chi_a}^2 + E_{chi_b}^2.
+
+    Every kernel splits as phi = u + i w with u = (phi + J phi)/2 and
+    w = (phi - J phi)/(2i) both J-real, and |E_phi|^2 = |E_u + i E_w|^2 =
+    E_u^2 + E_w^2, so the image is sum_a E_{x_a}^2 over the 2K J-real vectors
+    x = (u_1..u_K, w_1..w_K).  Their Gram matrix G = conj(X) X^T is real symmetric;
+    with G = Q diag(l) Q^T the vectors chi_j = sum_a Q_aj x_a are J-real,
+    orthogonal with |chi_j|^2 = l_j, and sum_j chi_j chi_j^H = sum_a x_a x_a^H,
+    so the image is sum_j E_{chi_j}^2.  Keeping the r eigenpairs that hold all
+    but rel_tol of trace(G) and pairing chi_{2j} + i chi_{2j+1} gives
+    ceil(r/2) kernels.
+
+    When the optics are symmetric under kappa -> -kappa with a pupil whose
+    phase is odd (no defocus or even aberrations) the TCC commutes with J,
+    r = K and the image costs half the kernels; otherwise r is usually 2K and
+    `ks` itself is returned.  float64 throughout (eigenpairs from the
+    library's Jacobi solver), result rounded to complex64 like the input."""
+    K, wr, wc = ks.K, ks.wr, ks.wc
+    if K < 2 or K > PAIR_MAX_K or 2 * ks.k0r + wr - 1 != 0 or 2 * ks.k0c + wc - 1 != 0:
+        return ks        # J maps the window onto itself only when it is centred
+    from .DecomposeTCC_SOCS import eigen_desc
+    phi = ks.win.to(torch.complex128).reshape(K, -1)
+    jphi = ks.win.flip(1, 2).conj().to(torch.complex128).reshape(K, -1)
+    X = torch.cat([(phi + jphi) * 0.5, (phi - jphi) * (-0.5j)], 0)
+    G = (X.conj() @ X.transpose(0, 1)).real
+    G = 0.5 * (G + G.transpose(0, 1))
+    lam, Q = eigen_desc(G.to(torch.complex128))
+    if float(Q.imag.abs().max()) > 1e-9:
+        return ks
+    lam = lam.clamp_min(0)
+    total = float(lam.sum())
+    if total <= 0:
+        return ks
+    tail = torch.flip(torch.cumsum(torch.flip(lam, [0]), 0), [0])      # tail[j] = sum_{i >= j}
+    keep = int((tail > rel_tol * total).sum())
+    npk = (keep + 1) // 2
+    if npk >= K:
+        return ks
+    chi = Q.real[:, :keep].transpose(0, 1).to(torch.complex128) @ X      # (keep, M), J-real rows
+    if keep % 2:
+        chi = torch.cat([chi, torch.zeros_like(chi[:1])], 0)
+    psi = chi[0::2] + 1j * chi[1::2]
+    win = psi.reshape(npk, wr, wc).to(torch.complex64).contiguous()
+    out = KernelSet(win, ks.k0r, ks.k0c, ks.ny, ks.nx, band=ks.band)
+    out._real = out
+    return out
 
 
 class Plan:
@@ -195,16 +263,20 @@ def plan_for(ks: KernelSet, nchan: int = 1, max_batch: int = 1) -> Plan:
 
 def socs_image(masks: torch.Tensor, ks: KernelSet, kind: int = nat.OI_FIELD_REAL, cin: int = 1,
                chans=(0,), out: torch.Tensor | None = None, max_batch: int | None = None,
-               plan: Plan | None = None) -> torch.Tensor:
+               plan: Plan | None = None, pack_real: bool = True) -> torch.Tensor:
     """Batched SOCS intensity (CalculateAerialImage_SOCS.py:3-11 per mask).
 
     masks: float32 [B, ny, nx] (kind REAL) or c64 [B, ny, nx, cin] on the
     device.  Returns float32 [B, ny, nx] in the reference's output layout
-    (after its final fftshift)."""
+    (after its final fftshift).  Real masks are imaged with
+    ks.for_real_masks() (the same image with up to half the kernels) unless
+    pack_real is False."""
     B = int(masks.shape[0])
     ny, nx = int(masks.shape[1]), int(masks.shape[2])
     if (ny, nx) != (ks.ny, ks.nx):
         raise ValueError(f"mask grid {ny}x{nx} does not match kernel grid {ks.ny}x{ks.nx}")
+    if kind == nat.OI_FIELD_REAL and pack_real:
+        ks = ks.for_real_masks()
     chans = list(chans)
     nchan = len(chans)
     if plan is None:
@@ -219,12 +291,15 @@ def socs_image(masks: torch.Tensor, ks: KernelSet, kind: int = nat.OI_FIELD_REAL
 
 
 def socs_adjoint(masks: torch.Tensor, grad_image: torch.Tensor, ks: KernelSet,
-                 out: torch.Tensor | None = None, plan: Plan | None = None) -> torch.Tensor:
+                 out: torch.Tensor | None = None, plan: Plan | None = None,
+                 pack_real: bool = True) -> torch.Tensor:
     """Mask gradient of the SOCS image: d/d masks of sum(grad_image * I(masks)).
 
     masks, grad_image: float32 [B, ny, nx] on the device (real masks).  The
     coherent fields are recomputed per kernel by the library (oi_socs_adjoint);
-    no forward state is reused."""
+    no forward state is reused.  The image is the same function of real masks
+    with ks.for_real_masks(), so its gradient is too; that set is used unless
+    pack_real is False."""
     B = int(masks.shape[0])
     ny, nx = int(masks.shape[1]), int(masks.shape[2])
     if (ny, nx) != (ks.ny, ks.nx):
@@ -235,6 +310,8 @@ def socs_adjoint(masks: torch.Tensor, grad_image: torch.Tensor, ks: KernelSet,
         raise ValueError("socs_adjoint takes float32 masks and image cotangents")
     masks = masks.contiguous()
     grad_image = grad_image.contiguous()
+    if pack_real:
+        ks = ks.for_real_masks()
     if plan is None:
         plan = plan_for(ks, 1, B)
     lib = nat.load()

@@ -86,6 +86,7 @@ def size_classes(sizes, source, projector, numerics, pixel_nm=4.0, focus=0.0, ma
     for n in sizes:
         period = n * pixel_nm
         ks = socs_kernel_set(source, [period, period], projector, focus, numerics, [n, n])
+        ks.for_real_masks()          # paired set for real masks, built before serving starts
         mb = (max_batch or {}).get(n, DEFAULT_MAX_BATCH.get(n, max(1, (1 << 22) // (n * n))))
         out.append(SizeClass(n, ks, mb))
     return out
@@ -120,6 +121,8 @@ class GpuExecutor:
         for c in classes.values():
             ks = c.kernels
             self.resident += ks.win.numel() * 8
+            if ks.for_real_masks() is not ks:
+                self.resident += ks.for_real_masks().win.numel() * 8
             br, bc = ks.band if ks.band is not None else (0, 0)
             for si in range(streams):
                 p = engine.Plan(ks.ny, ks.nx, ks.k0r, ks.wr, ks.k0c, ks.wc, nchan=1,

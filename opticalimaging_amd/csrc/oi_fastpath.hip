@@ -58,8 +58,8 @@ __global__ __launch_bounds__(kBlock) void k_fp_rows_real(const float* __restrict
 #pragma unroll
   for (int m = 0; m < V; ++m) {
     const int x = t + TT * m;
-    const float a = (live && r0 < ny) ? src[(size_t)r0 * N + x] : 0.f;
-    const float b = (live && r1 < ny) ? src[(size_t)r1 * N + x] : 0.f;
+    const float a = (live && r0 < ny) ? __builtin_nontemporal_load(src + (size_t)r0 * N + x) : 0.f;
+    const float b = (live && r1 < ny) ? __builtin_nontemporal_load(src + (size_t)r1 * N + x) : 0.f;
     v[m] = make_float2(a, b);
   }
   P::template run_live_tab<true>(v, scr, t, twt, true);
@@ -87,6 +87,116 @@ __global__ __launch_bounds__(kBlock) void k_fp_rows_real(const float* __restrict
   for (int e = threadIdx.x; e < wc * 2 * G; e += kBlock) {
     const int j = e / (2 * G), rr = e - j * 2 * G;
     if (rb + rr < ny) Tb[(size_t)j * ny + rb + rr] = tile[j * TS + rr];
+  }
+}
+
+// Row kernel for a window inside [-T, T] (T = P::T; every image plan has
+// U2 == 1).  After stage 1 and the exchange, thread t only needs outputs 0, 1
+// and V - 1 of its radix-V stage-2 DFT: X[t], X[t + T] and X[N - T + t], i.e.
+// kappa = t, T and t - T.  They are formed directly (one sum and two
+// twiddled sums) instead of the full transform, and the Hermitian partner
+// Z[-kappa] of the two packed rows comes from lane (T - t) mod T by a
+// cross-lane read, so the outputs never go back through LDS.  A block walks
+// kRowSets sets of 2G rows and issues the next set's loads before
+// transforming the current one, so the mask reads of one set overlap the
+// arithmetic of the other (the exchange scratch allows two blocks per CU).
+constexpr int kRowSets = 2;
+
+// Rows r0 and r0 + 1 (the caller guarantees both exist).  The mask is read
+// once per pass: non-temporal loads keep it from displacing the pass's
+// intermediates in L2 and the last-level cache, and from waiting on the
+// write-back of dirty lines the previous pass's image left there.
+template <class P>
+__device__ __forceinline__ void rows_load_pair(float2 (&v)[P::V], const float* __restrict__ src,
+                                               int r0, int t) {
+  const float* a = src + (size_t)r0 * P::N + t;
+#pragma unroll
+  for (int m = 0; m < P::V; ++m)
+    v[m] = make_float2(__builtin_nontemporal_load(a + P::T * m),
+                       __builtin_nontemporal_load(a + P::N + P::T * m));
+}
+
+__device__ __forceinline__ void rows_separate(float2 zk, float2 zm, float2* dst) {
+  dst[0] = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));     // (Z[k] + conj Z[-k]) / 2
+  dst[1] = make_float2(0.5f * (zk.y + zm.y), -0.5f * (zk.x - zm.x));    // (Z[k] - conj Z[-k]) / 2i
+}
+
+template <class P>
+__global__ __launch_bounds__(kBlock) void k_fp_rows_real_narrow(const float* __restrict__ mask,
+                                                                int ny, int k0c, int wc,
+                                                                const float2* __restrict__ twt,
+                                                                float2* __restrict__ T) {
+  constexpr int N = P::N, TT = P::T, V = P::V, R1 = P::R1, R2 = P::R2, U1 = P::U1;
+  constexpr int G = groups_per_block<P>();
+  static_assert(P::U2 == 1 && R2 == V && TT == R1 && kBlock % TT == 0 && (TT & (TT - 1)) == 0 &&
+                    TT <= 64,
+                "narrow rows: radix-V second stage, power-of-two groups inside a wavefront");
+  // ny is a multiple of 2G (checked by the launch): every set is whole.
+  extern __shared__ float2 lds[];
+  const int g = threadIdx.x / TT, t = threadIdx.x - g * TT;
+  float2* scr = lds + g * P::LDS;
+  constexpr int TS = 2 * G + 1;
+  float2* tile = lds + G * P::LDS;                 // [wc][TS]
+  const int img = blockIdx.y;
+  const float* src = mask + (size_t)img * ny * N;
+  float2* Tb = T + (size_t)img * wc * ny;
+  const int lane = threadIdx.x & 63;
+  const int partner = (lane & ~(TT - 1)) | ((TT - t) & (TT - 1));
+  const int sets = (ny + 2 * G - 1) / (2 * G);
+  float2 nxt[V];
+  rows_load_pair<P>(nxt, src, 2 * (blockIdx.x * kRowSets * G + g), t);
+  for (int it = 0; it < kRowSets; ++it) {
+    const int set = blockIdx.x * kRowSets + it;
+    if (set >= sets) break;                        // uniform over the block
+    float2 v[V];
+#pragma unroll
+    for (int m = 0; m < V; ++m) v[m] = nxt[m];
+    if (it + 1 < kRowSets && set + 1 < sets) rows_load_pair<P>(nxt, src, 2 * ((set + 1) * G + g), t);
+    // stage 1 (as GFFT::run_gen), outputs to the exchange scratch
+#pragma unroll
+    for (int u = 0; u < U1; ++u) {
+      float2 b[R1];
+#pragma unroll
+      for (int j = 0; j < R1; ++j) b[j] = v[u + j * U1];
+      reg::DFT<R1, true>::run(b);
+      const int i = t + TT * u;
+#pragma unroll
+      for (int j = 0; j < R1; ++j) scr[P::xi(i * R1 + j)] = b[j];
+    }
+    __syncthreads();
+    // stage 2, outputs 0, 1 and V - 1 only: sum_j b_j e^{+2 pi i j k / V}
+    float sx = 0.f, sy = 0.f, A = 0.f, B = 0.f, C = 0.f, D = 0.f;
+#pragma unroll
+    for (int j = 0; j < V; ++j) {
+      float2 b = scr[P::xi(t + j * R1)];
+      if (j > 0) {
+        const float2 w = twt[j * t];              // exp(-2 pi i j t / N), used conjugated
+        b = reg::mul(b, make_float2(w.x, -w.y));
+      }
+      const float c = (float)reg::root_re(j, V), s = (float)(-reg::root_im(j, V));
+      sx += b.x;
+      sy += b.y;
+      A = fmaf(b.x, c, A);
+      B = fmaf(b.y, s, B);
+      C = fmaf(b.x, s, C);
+      D = fmaf(b.y, c, D);
+    }
+    const float2 o0 = make_float2(sx, sy);          // X[t]
+    const float2 o1 = make_float2(A - B, C + D);    // X[t + T]
+    const float2 om = make_float2(A + B, D - C);    // X[N - T + t]
+    const float2 p0 = make_float2(__shfl(o0.x, partner), __shfl(o0.y, partner));
+    const float2 pm = make_float2(__shfl(om.x, partner), __shfl(om.y, partner));
+    const int ja = t - k0c, jb = t - TT - k0c, jc = TT - k0c;
+    if (ja >= 0 && ja < wc) rows_separate(o0, t == 0 ? o0 : pm, tile + ja * TS + 2 * g);   // kappa = t
+    if (jb >= 0 && jb < wc) rows_separate(om, t == 0 ? o1 : p0, tile + jb * TS + 2 * g);   // t - T
+    if (t == 0 && jc >= 0 && jc < wc) rows_separate(o1, om, tile + jc * TS + 2 * g);       // T
+    __syncthreads();
+    const int rb = 2 * set * G;
+    for (int e = threadIdx.x; e < wc * 2 * G; e += kBlock) {
+      const int j = e / (2 * G), rr = e - j * 2 * G;
+      Tb[(size_t)j * ny + rb + rr] = tile[j * TS + rr];
+    }
+    __syncthreads();                               // tile and scratch are reused
   }
 }
 
@@ -623,12 +733,14 @@ __global__ __launch_bounds__(kBlock) void k_fp_urows(const float2* __restrict__ 
   }
   P::template run_live_tab<false>(v, scr + g * P::LDS, t, twt, true);
   if (!live) return;
+  // the image is written once and not read by this pass: non-temporal stores
+  // stream it to memory instead of leaving dirty lines in the caches
   const int ya = y0 + 2 * gg, yb = ya + 1;
 #pragma unroll
   for (int m = 0; m < V; ++m) {
     const int x = t + TT * m;
-    if (ya < ny) out[((size_t)img * ny + ya) * N + x] = v[m].x;
-    if (yb < ny) out[((size_t)img * ny + yb) * N + x] = v[m].y;
+    if (ya < ny) __builtin_nontemporal_store(v[m].x, out + ((size_t)img * ny + ya) * N + x);
+    if (yb < ny) __builtin_nontemporal_store(v[m].y, out + ((size_t)img * ny + yb) * N + x);
   }
 }
 
@@ -661,7 +773,14 @@ static int run_rows(const FpArgs& a, hipStream_t st) {
   constexpr int G = groups_per_block<P>();
   // transform scratch + output tile [wc][2G + 1]
   const size_t lds = rows_lds<P>() + (size_t)a.wc * (2 * G + 1) * sizeof(float2);
-  if (a.kind == OI_FIELD_REAL) {
+  if (a.kind == OI_FIELD_REAL && a.k0c >= -P::T && a.k0c + a.wc - 1 <= P::T && a.ny % (2 * G) == 0) {
+    int rc = allow(k_fp_rows_real_narrow<P>, lds);
+    if (rc) return rc;
+    const int sets = (a.ny + 2 * G - 1) / (2 * G);
+    dim3 grid((sets + kRowSets - 1) / kRowSets, a.batch);
+    hipLaunchKernelGGL(k_fp_rows_real_narrow<P>, grid, dim3(kBlock), lds, st,
+                       static_cast<const float*>(a.field), a.ny, a.k0c, a.wc, a.tw_x, a.T);
+  } else if (a.kind == OI_FIELD_REAL) {
     int rc = allow(k_fp_rows_real<P>, lds);
     if (rc) return rc;
     dim3 grid((a.ny + 2 * G - 1) / (2 * G), a.batch);

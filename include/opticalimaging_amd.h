@@ -168,6 +168,16 @@ int oi_kernel_window(const void* dense, int ny, int nx, int nk, int k0r, int wr,
 int oi_kernel_dense(const void* win, int nk, int k0r, int wr, int k0c, int wc,
                     int ny, int nx, void* dense, void* stream);
 
+/* Masks sent as one byte per pixel (serving): out[i] = levels[codes[i]] for
+ * i < count, float32; codes >= nlevels give 0.  levels is a host array of
+ * nlevels <= 256 floats (copied into the launch); codes and out are device
+ * pointers aligned to 16 bytes.  No counterpart in the reference, whose masks
+ * are complex ScatterField tensors converted at use (Scatter.py:13-14); a
+ * two-level attenuated-PSM clip crosses PCIe in a quarter of its float32
+ * bytes this way.                                                           */
+int oi_mask_decode_u8(const void* codes, size_t count, const float* levels, int nlevels,
+                      float* out, void* stream);
+
 /* ----------------------------------------------------------- pupil / TCC --
  * Optics of one focus plane (Projection.py:5-10, Source.py:18).            */
 typedef struct oi_optics {

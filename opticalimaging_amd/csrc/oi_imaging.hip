@@ -573,3 +573,55 @@ extern "C" int oi_kernel_dense(const void* win, int nk, int k0r, int wr, int k0c
   OI_LAUNCH_CHECK("kernel_dense");
   return OI_OK;
 }
+
+// ------------------------------------------------------------ coded masks --
+// Masks sent as one byte per pixel: out[i] = levels[codes[i]] (codes past the
+// table give 0).  Sixteen codes per thread and iteration (one 16-byte load,
+// four 16-byte stores); the table sits in LDS.
+namespace {
+struct Levels {
+  float v[256];
+};
+
+__global__ __launch_bounds__(256) void k_decode_u8(const uint8_t* __restrict__ codes, size_t n,
+                                                   Levels lv, int nl, float* __restrict__ out) {
+  __shared__ float lut[256];
+  lut[threadIdx.x] = (int)threadIdx.x < nl ? lv.v[threadIdx.x] : 0.f;
+  __syncthreads();
+  const size_t n16 = n / 16;
+  const uint4* c16 = reinterpret_cast<const uint4*>(codes);
+  float4* o4 = reinterpret_cast<float4*>(out);
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride) {
+    const uint4 c = c16[i];
+    const uint32_t w[4] = {c.x, c.y, c.z, c.w};
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      o4[4 * i + q] = make_float4(lut[w[q] & 0xff], lut[(w[q] >> 8) & 0xff], lut[(w[q] >> 16) & 0xff],
+                                  lut[w[q] >> 24]);
+  }
+  if (blockIdx.x == 0) {
+    for (size_t i = n16 * 16 + threadIdx.x; i < n; i += blockDim.x) out[i] = lut[codes[i]];
+  }
+}
+}  // namespace
+
+extern "C" int oi_mask_decode_u8(const void* codes, size_t count, const float* levels, int nlevels,
+                                 float* out, void* stream) {
+  if (!codes || !out || !levels || nlevels < 1 || nlevels > 256)
+    return fail(OI_ERR_INVALID, "mask_decode_u8: null pointer or level count outside 1..256");
+  if ((reinterpret_cast<uintptr_t>(codes) & 15) || (reinterpret_cast<uintptr_t>(out) & 15))
+    return fail(OI_ERR_INVALID, "mask_decode_u8: codes and out must be 16-byte aligned");
+  if (count == 0) return OI_OK;
+  Levels lv;
+  for (int i = 0; i < 256; ++i) lv.v[i] = i < nlevels ? levels[i] : 0.f;
+  const size_t n16 = count / 16;
+  size_t blocks = (n16 + 255) / 256;
+  if (blocks < 1) blocks = 1;
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(k_decode_u8, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,
+                     static_cast<const uint8_t*>(codes), count, lv, nlevels, out);
+  OI_LAUNCH_CHECK("mask_decode_u8");
+  return OI_OK;
+}
+

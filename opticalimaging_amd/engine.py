@@ -322,3 +322,22 @@ def socs_adjoint(masks: torch.Tensor, grad_image: torch.Tensor, ks: KernelSet,
     nat.check(lib.oi_socs_adjoint(plan.handle, nat.ptr(masks), nat.ptr(grad_image), B,
                                   nat.ptr(ks.win), ks.K, nat.ptr(work), nat.ptr(out), nat.stream()))
     return out
+
+
+def decode_mask_codes(codes: torch.Tensor, levels, out: torch.Tensor | None = None) -> torch.Tensor:
+    """float32 masks levels[codes] from uint8 codes on the device
+    (oi_mask_decode_u8): masks with at most 256 distinct values cross the host
+    link as one byte per pixel.  `levels` is a sequence of floats."""
+    if codes.dtype != torch.uint8 or not codes.is_cuda:
+        raise ValueError("mask codes must be a uint8 tensor on the device")
+    codes = codes.contiguous()
+    if out is None:
+        out = torch.empty(codes.shape, dtype=torch.float32, device=codes.device)
+    if out.dtype != torch.float32 or out.numel() != codes.numel() or not out.is_contiguous():
+        raise ValueError("out must be a contiguous float32 tensor with one value per code")
+    lv = [float(v) for v in levels]
+    arr = (ctypes.c_float * max(1, len(lv)))(*lv)
+    nat.check(nat.load().oi_mask_decode_u8(nat.ptr(codes), codes.numel(), arr, len(lv), nat.ptr(out),
+                                           nat.stream()))
+    return out
+

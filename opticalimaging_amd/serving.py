@@ -4,7 +4,8 @@ Requests are mask clips of several sizes (one size class per grid size, each
 with its own device-resident kernel set).  One host thread runs the loop:
 
   * arrivals join a FIFO; a request is *admitted* when its device bytes (mask
-    in + image out, float32) fit in what the HBM budget leaves after the
+    in + image out, float32, plus one byte per pixel when the mask arrives as
+    codes into a table of levels) fit in what the HBM budget leaves after the
     resident kernel sets and plan scratch;
   * admitted requests wait per size class; whenever one of S execution slots
     (a HIP stream with its own plans) is free, the class holding the oldest
@@ -33,18 +34,24 @@ import numpy as np
 import torch
 
 __all__ = ["Request", "Completed", "SizeClass", "Server", "GpuExecutor", "latency_stats",
-           "size_classes", "pinned_masks"]
+           "size_classes", "pinned_masks", "pinned_codes"]
 
 
 @dataclass
 class Request:
     rid: int
-    mask: object            # (n, n) float32 host array (numpy or pinned torch)
+    mask: object            # (n, n) float32 host array (numpy or pinned torch), or uint8
+                            # codes into `levels`
     arrival: float = 0.0    # seconds from the start of the run
+    levels: tuple | None = None   # pixel value of each code when `mask` holds uint8 codes
 
     @property
     def n(self) -> int:
         return int(self.mask.shape[0])
+
+    @property
+    def coded(self) -> bool:
+        return self.levels is not None
 
 
 @dataclass
@@ -73,6 +80,11 @@ class SizeClass:
     def request_bytes(self) -> int:
         return 2 * 4 * self.n * self.n
 
+    def bytes_for(self, req: "Request") -> int:
+        """Device bytes of one request: float32 mask and image, plus the byte
+        codes when the mask arrives coded."""
+        return self.request_bytes + (self.n * self.n if req.coded else 0)
+
 
 DEFAULT_MAX_BATCH = {128: 64, 256: 64, 512: 32, 1024: 16, 2048: 4}
 
@@ -89,6 +101,22 @@ def size_classes(sizes, source, projector, numerics, pixel_nm=4.0, focus=0.0, ma
         ks.for_real_masks()          # paired set for real masks, built before serving starts
         mb = (max_batch or {}).get(n, DEFAULT_MAX_BATCH.get(n, max(1, (1 << 22) // (n * n))))
         out.append(SizeClass(n, ks, mb))
+    return out
+
+
+def pinned_codes(rids, sizes, make):
+    """Coded request masks in one pinned host allocation: make(rid, n) ->
+    (uint8 codes (n, n), levels); returns a list of (pinned codes, levels)."""
+    sizes = [int(n) for n in sizes]
+    pool = torch.empty(sum(n * n for n in sizes), dtype=torch.uint8, pin_memory=True)
+    out = []
+    off = 0
+    for rid, n in zip(rids, sizes):
+        t = pool[off:off + n * n].view(n, n)
+        codes, levels = make(rid, n)
+        t.numpy()[...] = codes
+        out.append((t, tuple(float(v) for v in levels)))
+        off += n * n
     return out
 
 
@@ -140,15 +168,27 @@ class GpuExecutor:
         b = len(reqs)
         n = cls.n
         srcs = [torch.as_tensor(r.mask) for r in reqs]
+        keep = [srcs]
         with torch.cuda.stream(s):
             dev_in = torch.empty((b, n, n), dtype=torch.float32, device="cuda")
-            for i, m in enumerate(srcs):      # pinned masks: asynchronous copies on s
-                dev_in[i].copy_(m, non_blocking=m.is_pinned())
+            coded = [i for i, r in enumerate(reqs) if r.coded]
+            if coded:
+                codes = torch.empty((b, n, n), dtype=torch.uint8, device="cuda")
+                keep.append(codes)
+            for i, m in enumerate(srcs):      # pinned host memory: asynchronous copies on s
+                dst = codes[i] if reqs[i].coded else dev_in[i]
+                dst.copy_(m, non_blocking=m.is_pinned())
+            if coded and len(coded) == b and all(r.levels == reqs[0].levels for r in reqs):
+                self.engine.decode_mask_codes(codes, reqs[0].levels, out=dev_in)
+            else:
+                for i in coded:
+                    self.engine.decode_mask_codes(codes[i], reqs[i].levels, out=dev_in[i])
             out = self.engine.socs_image(dev_in, cls.kernels, out=torch.empty_like(dev_in),
                                          plan=self.plans[(n, slot)])
             ev = torch.cuda.Event()
             ev.record(s)
-        return {"event": ev, "out": out, "keep": (srcs, dev_in)}
+        keep.append(dev_in)
+        return {"event": ev, "out": out, "keep": keep}
 
     def done(self, handle) -> bool:
         return handle["event"].query()
@@ -171,6 +211,7 @@ class _Batch:
 class _Entry:
     req: Request
     admitted: float = 0.0
+    nbytes: int = 0
 
 
 class Server:
@@ -187,6 +228,7 @@ class Server:
         for c in self.classes.values():
             if c.request_bytes > self.budget - self.resident:
                 raise ValueError(f"a {c.n}^2 request does not fit in the HBM budget")
+        self.room = self.budget - self.resident
         self.in_use = 0.0
         self.peak_in_use = 0.0
         self.waiting = collections.deque()
@@ -204,7 +246,7 @@ class Server:
                 for e, img in zip(b.reqs, outs):
                     self.completed.append(Completed(e.req.rid, e.req.n, img, e.req.arrival,
                                                     e.admitted, b.launched, now, len(b.reqs)))
-                    self.in_use -= b.cls.request_bytes
+                    self.in_use -= e.nbytes
                 self.free_slots.append(b.slot)
             else:
                 still.append(b)
@@ -213,13 +255,15 @@ class Server:
     def _admit(self, now):
         while self.waiting:
             r = self.waiting[0]
-            c = self.classes[r.n]
-            if self.resident + self.in_use + c.request_bytes > self.budget:
+            nb = self.classes[r.n].bytes_for(r)
+            if nb > self.room:
+                raise ValueError(f"request {r.rid} ({nb} B) does not fit in the HBM budget")
+            if self.resident + self.in_use + nb > self.budget:
                 break
             self.waiting.popleft()
-            self.in_use += c.request_bytes
+            self.in_use += nb
             self.peak_in_use = max(self.peak_in_use, self.in_use)
-            self.pending[r.n].append(_Entry(r, now))
+            self.pending[r.n].append(_Entry(r, now, nb))
 
     def _launch(self, now):
         while self.free_slots:
@@ -246,6 +290,8 @@ class Server:
         for r in requests:
             if self.classes.get(r.n) is None:
                 raise ValueError(f"no size class for {r.n}^2 requests")
+            if self.classes[r.n].bytes_for(r) > self.room:
+                raise ValueError(f"request {r.rid} does not fit in the HBM budget")
         clock = clock or time.perf_counter
         t0 = clock()
         i = 0

@@ -217,3 +217,13 @@ def serving_arrivals(count, rate, sizes=SERVING_SIZES, seed=17):
 
 def serving_mask(rid, n):
     return att_psm_mask(n, 4.0 * n, seed=int(rid))
+
+
+ATT_PSM_LEVELS = (1.0, float(np.float32(ATT_PSM_AMPLITUDE)))
+
+
+def serving_mask_codes(rid, n):
+    """The same clip as serving_mask as (uint8 codes, levels): code 1 marks
+    absorber pixels; levels[code] is bit-identical to serving_mask's float32."""
+    rng = np.random.default_rng(int(rid))
+    return manhattan(n, 4.0 * n, rng).astype(np.uint8), ATT_PSM_LEVELS

@@ -49,3 +49,42 @@ def test_mix_equals_alone_and_oracle(gpu):
         err = rel_err(mixed[i], ref)
         print(f"served {n}^2 request {i} vs float64 SOCS: {err:.2e}")
         assert err < 1e-5
+
+
+def test_mask_decode_kernel(gpu):
+    """oi_mask_decode_u8: levels[codes] exactly, any length (16-code body and
+    tail), codes past the table give 0, misaligned pointers are refused."""
+    from opticalimaging_amd import engine
+    rng = np.random.default_rng(1)
+    levels = (0.25, -1.5, 3.0)
+    for count in (1, 15, 16, 1000, 4096 * 16 + 7):
+        codes = torch.as_tensor(rng.integers(0, 4, count).astype(np.uint8)).cuda()
+        out = engine.decode_mask_codes(codes, levels)
+        lut = torch.tensor(levels + (0.0,), dtype=torch.float32)
+        assert torch.equal(out.cpu(), lut[codes.cpu().long()]), count
+    buf = torch.zeros(64, dtype=torch.uint8, device="cuda")
+    with pytest.raises(ValueError):
+        engine.decode_mask_codes(buf[1:33], levels)
+
+
+def test_coded_requests_equal_float_requests(gpu):
+    """The same clips sent as byte codes and as float32 give bit-identical
+    images (the decoded mask is the float32 mask exactly)."""
+    from opticalimaging_amd import serving, workloads
+    src, proj, num = workloads.ilt_optics()
+    classes = serving.size_classes((128, 256), src, proj, num, max_batch={128: 4, 256: 4})
+    t, sizes = workloads.serving_arrivals(24, rate=50000.0, sizes=(128, 256), seed=9)
+    masks = serving.pinned_masks(range(24), sizes, workloads.serving_mask)
+    codes = serving.pinned_codes(range(24), sizes, workloads.serving_mask_codes)
+    ex = serving.GpuExecutor({c.n: c for c in classes}, streams=2)
+    budget = ex.resident + 64 * 2**20
+    a = serving.Server(classes, ex, hbm_budget=budget).run(
+        [serving.Request(i, masks[i], float(t[i])) for i in range(24)])
+    b = serving.Server(classes, ex, hbm_budget=budget).run(
+        [serving.Request(i, codes[i][0], float(t[i]), levels=codes[i][1]) for i in range(24)])
+    torch.cuda.synchronize()
+    ia = {c.rid: c.image.cpu() for c in a}
+    ib = {c.rid: c.image.cpu() for c in b}
+    worst = max(float((ia[i] - ib[i]).abs().max()) for i in range(24))
+    print("max |float - coded| over 24 requests:", worst)
+    assert worst == 0.0

@@ -140,3 +140,25 @@ def test_budget_too_small_raises():
         Server([SizeClass(2048, None, 4)], SimExecutor(clock), hbm_budget=1e6)
     with pytest.raises(ValueError):
         Server([SizeClass(256, None, 4)], SimExecutor(clock, resident=2e9), hbm_budget=1e9)
+
+
+def test_coded_requests_count_their_code_bytes():
+    """A request whose mask arrives as byte codes holds one more byte per
+    pixel on the device until it completes."""
+    clock = [0.0]
+    ex = SimExecutor(clock, slots=4)
+    cls = SizeClass(1024, None, 16)
+    coded = Request(0, np.zeros((1024, 1024), np.uint8), 0.0, levels=(1.0, -0.25))
+    plain = Request(1, np.zeros((1024, 1024), np.float32), 0.0)
+    assert cls.bytes_for(coded) == cls.request_bytes + 1024 * 1024
+    assert cls.bytes_for(plain) == cls.request_bytes
+    per = cls.bytes_for(coded)
+    srv = Server([cls], ex, hbm_budget=2 * per + 10)        # two coded requests fit
+    reqs = [Request(i, np.zeros((1024, 1024), np.uint8), 0.0, levels=(1.0, 0.0)) for i in range(9)]
+    inflight_max = _drive(srv, ex, reqs)
+    assert inflight_max <= 2 and srv.peak_in_use <= 2 * per
+    assert srv.in_use == 0 and len(srv.completed) == 9
+    # a coded clip that cannot fit even alone is refused before serving starts
+    tight = Server([cls], SimExecutor(clock), hbm_budget=cls.request_bytes + 10)
+    with pytest.raises(ValueError):
+        tight.run([coded])

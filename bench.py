@@ -59,7 +59,7 @@ def parse():
                     help="masks in the CPU K=24 SOCS sample (a third of them for CPU Abbe)")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip accuracy, plain-Abbe, TCC and CPU legs (profiling runs)")
-    ap.add_argument("--serving-requests", type=int, default=1200,
+    ap.add_argument("--serving-requests", type=int, default=10000,
                     help="requests in the serving leg (0 skips it)")
     ap.add_argument("--serving-rate", type=float, default=3000.0,
                     help="offered Poisson rate of the serving leg (requests/s)")
@@ -206,12 +206,12 @@ def serving_leg(count, rate):
     serving.Server(classes, ex, hbm_budget=budget).run([coded(i, 0.0) for i in range(min(count, 16))])
     torch.cuda.synchronize()
     # capacity: the first 256 clips all present at t = 0, coded and as float32
-    srv = serving.Server(classes, ex, hbm_budget=budget)
+    srv = serving.Server(classes, ex, hbm_budget=budget, keep_images=False)
     cap = serving.latency_stats(srv.run([coded(i, 0.0) for i in range(nb)]))
-    srv = serving.Server(classes, ex, hbm_budget=budget)
+    srv = serving.Server(classes, ex, hbm_budget=budget, keep_images=False)
     cap_f32 = serving.latency_stats(srv.run([serving.Request(i, masks[i], 0.0) for i in range(nb)]))
-    # offered load
-    srv = serving.Server(classes, ex, hbm_budget=budget)
+    # offered load (images are dropped as they complete)
+    srv = serving.Server(classes, ex, hbm_budget=budget, keep_images=False)
     done = srv.run([coded(i, float(arr[i])) for i in range(count)])
     st = serving.latency_stats(done)
     per = {}
@@ -284,6 +284,17 @@ def ilt_leg(steps):
             "images_per_s": round(16 * steps / dt, 1),
             "hbm_alg_GBps": round(b_alg / (dt / steps) / 1e9, 1),
             "loss_first": float(lo[0]), "loss_last": float(lo[-1])}
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for ln in fh:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def main():
@@ -487,26 +498,42 @@ def main():
             O.socs_image(workloads.field_from_mask(masks_np[i]).numpy(), kern64, period, period,
                          float(proj.NA), float(src.Wavelength))
         dt24 = time.perf_counter() - t
-        na = max(1, args.cpu_sample // 3)
-        t = time.perf_counter()
+        na = 5
+        per = []
         ref_abbe = None
         for i in range(na):
             sc.ScatterField = workloads.field_from_mask(masks_np[i]).numpy()
+            t = time.perf_counter()
             r, *_ = O.abbe_image(src, sc, proj, num)
+            per.append(time.perf_counter() - t)
             if i == 0:
                 ref_abbe = r
-        dt = time.perf_counter() - t
-        line["cpu_baseline"] = {"value": round(na / dt, 4), "unit": "images/s", "cores": 1,
+        dt = float(np.sum(per))
+        med = float(np.median(per))
+        # configuration 1 in full: the 128^2 scalar Abbe image, 1 warm-up + 5 runs
+        c1_mask, c1_src, c1_proj, c1_num, c1_period = workloads.scalar_abbe_128()
+        c1 = type("S", (), {})()
+        c1.Period_X = c1.Period_Y = c1_period
+        c1.ScatterField = workloads.field_from_mask(c1_mask).numpy()
+        c1_t = []
+        for i in range(6):
+            t = time.perf_counter()
+            O.abbe_image(c1_src, c1, c1_proj, c1_num)
+            c1_t.append(time.perf_counter() - t)
+        line["cpu_baseline"] = {"value": round(1.0 / med, 4), "unit": "images/s", "cores": 1,
                                 "kind": "port",
                                 "sample": f"{na} of the 64 masks through the float64 numpy restatement "
                                           "of CalculateAbbeImage (120 source points, one full "
-                                          "1024^2 FFT each; equal to full-order SOCS), one thread",
+                                          "1024^2 FFT each; equal to full-order SOCS), one thread; "
+                                          "value = 1 / median seconds per image",
+                                "abbe_1024_s_per_image": [round(x, 3) for x in per],
+                                "c1_abbe_128_s_median": round(float(np.median(c1_t[1:])), 4),
                                 "k24_socs_images_per_s": round(ns / dt24, 4),
                                 "k24_socs_sample": f"{ns} masks, float64 restatement of "
                                                    "CalculateAerialImage_SOCS with the 24 kernels "
                                                    f"(kernel build {t_setup:.1f}s not included)",
-                                "host_cpus": os.cpu_count()}
-        log(f"cpu baseline: Abbe {na / dt:.4f} images/s over {dt:.1f}s; SOCS K=24 "
+                                "host_cpus": os.cpu_count(), "host_cpu_model": cpu_model()}
+        log(f"cpu baseline: Abbe {1.0 / med:.4f} images/s (median of {na}, {dt:.1f}s); SOCS K=24 "
             f"{ns / dt24:.3f} images/s")
         err_full = rel_err(gpu_img0, ref_abbe[0])
         err_socs = rel_err(gpu24_img0, ref_socs[0])

@@ -218,7 +218,7 @@ class Server:
     """Continuous batching under an HBM budget (bytes, including the resident
     kernel sets and plan scratch reported by the executor)."""
 
-    def __init__(self, classes, executor, hbm_budget: float = 8e9):
+    def __init__(self, classes, executor, hbm_budget: float = 8e9, keep_images: bool = True):
         self.classes = {c.n: c for c in classes} if not isinstance(classes, dict) else classes
         self.ex = executor
         self.budget = float(hbm_budget)
@@ -236,13 +236,14 @@ class Server:
         self.running = []
         self.free_slots = list(range(executor.slots))
         self.completed = []
+        self.keep_images = keep_images      # False: Completed.image is None (timing runs)
 
     # -- one pass of the loop -------------------------------------------------
     def _complete(self, now):
         still = []
         for b in self.running:
             if self.ex.done(b.handle):
-                outs = self.ex.outputs(b.handle)
+                outs = self.ex.outputs(b.handle) if self.keep_images else [None] * len(b.reqs)
                 for e, img in zip(b.reqs, outs):
                     self.completed.append(Completed(e.req.rid, e.req.n, img, e.req.arrival,
                                                     e.admitted, b.launched, now, len(b.reqs)))

@@ -12,7 +12,10 @@ K = 24 (the configuration's stated order) is reported beside it.
 A step images the whole batch (masks already in HBM -> intensities in HBM).
 Also reported, outside the timed region:
   * per-launch durations of the SOCS pass from HIP events recorded on the
-    launch stream during the timed steps, and the HBM roofline of the pass;
+    launch stream between the launches, in a repeat of the timed steps (the
+    events stay out of the timed loop), and the HBM roofline of the pass;
+  * the same batch issued twice on two streams (throughput of consecutive
+    batches whose stages overlap);
   * accuracy of image 0 against the float64 SOCS and float64 Abbe oracles;
   * the plain GPU Abbe baseline (per-source torch.fft restatement of the
     reference's CalculateAbbeImage) on the same masks, and the speed-up;
@@ -347,7 +350,6 @@ def main():
         for _ in range(warmup):
             engine.socs_image(masks, ks, out=out, plan=plan)
         torch.cuda.synchronize()
-        plan.set_timing(steps)
         stream = torch.cuda.current_stream()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         torch.cuda.synchronize()
@@ -358,6 +360,12 @@ def main():
         e1.record(stream)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
+        # per-stage durations: the same steps again with HIP events recorded
+        # between the launches (kept out of the timed loop above)
+        plan.set_timing(steps)
+        for _ in range(steps):
+            engine.socs_image(masks, ks, out=out, plan=plan)
+        torch.cuda.synchronize()
         stage_ms, ncalls = plan.read_timing()
         plan.set_timing(0)
         stage_avg = {k: v / ncalls for k, v in stage_ms.items()}
@@ -376,6 +384,41 @@ def main():
     achieved = b_alg / (pass_ms * 1e-3) / 1e9
     dominant = max(stage_avg, key=stage_avg.get)
     out24, wall24, stage24, info24 = run(ks24, args.steps, args.warmup)
+
+    def two_streams(ks, steps):
+        """Consecutive batches alternating over two streams, each with its own
+        plan and output, so that one batch's compute-bound stages overlap the
+        other's streaming stages."""
+        br, bc = ks.band if ks.band is not None else (0, 0)
+        plans = [engine.Plan(ks.ny, ks.nx, ks.k0r, ks.wr, ks.k0c, ks.wc, nchan=1, max_batch=B,
+                             band_r=min(br, ks.wr), band_c=min(bc, ks.wc)) for _ in range(2)]
+        outs = [torch.empty_like(masks) for _ in range(2)]
+        streams = [torch.cuda.Stream() for _ in range(2)]
+
+        def issue(n):
+            cur = torch.cuda.current_stream()
+            for s_ in streams:
+                s_.wait_stream(cur)
+            for i in range(n):
+                with torch.cuda.stream(streams[i % 2]):
+                    engine.socs_image(masks, ks, out=outs[i % 2], plan=plans[i % 2])
+            for s_ in streams:
+                cur.wait_stream(s_)
+
+        issue(4)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        issue(steps)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        same = bool(torch.equal(outs[0], out) and torch.equal(outs[1], out))
+        for p_ in plans:
+            p_.close()
+        return {"images_per_s": round(B * steps / dt, 2), "ms_per_batch": round(dt / steps * 1e3, 4),
+                "images_equal_single_stream": same}
+
+    two = two_streams(ks_full, 2 * (args.steps // 2))
+    log(f"two streams, consecutive batches: {json.dumps(two)}")
 
     line = {
         "metric": METRIC,
@@ -412,6 +455,7 @@ def main():
                      "avg_launch_ms": round(pass_ms, 5),
                      "stage_avg_ms": {k: round(v, 5) for k, v in stage_avg.items()},
                      "dominant_stage": dominant},
+        "two_stream_batches": two,
         "socs_k24": {"images_per_s": round(B * args.steps / wall24, 2),
                      "kernels_per_pass": pk24.K,
                      "ms_per_step": round(wall24 / args.steps * 1e3, 4),

@@ -275,6 +275,9 @@ def socs_image(masks: torch.Tensor, ks: KernelSet, kind: int = nat.OI_FIELD_REAL
     ny, nx = int(masks.shape[1]), int(masks.shape[2])
     if (ny, nx) != (ks.ny, ks.nx):
         raise ValueError(f"mask grid {ny}x{nx} does not match kernel grid {ks.ny}x{ks.nx}")
+    if B == 0:
+        return out if out is not None else torch.empty((0, ny, nx), dtype=torch.float32,
+                                                       device=masks.device)
     if kind == nat.OI_FIELD_REAL and pack_real:
         ks = ks.for_real_masks()
     chans = list(chans)
@@ -308,6 +311,9 @@ def socs_adjoint(masks: torch.Tensor, grad_image: torch.Tensor, ks: KernelSet,
         raise ValueError("grad_image must have the shape of masks")
     if masks.dtype != torch.float32 or grad_image.dtype != torch.float32:
         raise ValueError("socs_adjoint takes float32 masks and image cotangents")
+    if B == 0:
+        return out if out is not None else torch.empty((0, ny, nx), dtype=torch.float32,
+                                                       device=masks.device)
     masks = masks.contiguous()
     grad_image = grad_image.contiguous()
     if pack_real:

@@ -161,3 +161,14 @@ def test_batch_independence_and_partial_batches(gpu):
     assert torch.equal(full[2:5], part) and torch.equal(full[3], alone[0])
     with pytest.raises(ValueError):
         engine.socs_image(torch.zeros((9, n, n), device="cuda"), ks, plan=plan)
+
+
+def test_empty_batch_is_a_no_op(gpu):
+    """An empty batch returns an empty image (and gradient) without a launch."""
+    from opticalimaging_amd import engine
+    kern = np.zeros((64, 64, 2), dtype=np.complex64)
+    kern[0, 0, 0] = kern[1, 0, 1] = 1.0
+    ks = engine.KernelSet.from_dense(torch.as_tensor(kern))
+    m = torch.zeros((0, 64, 64), device="cuda")
+    assert engine.socs_image(m, ks).shape == (0, 64, 64)
+    assert engine.socs_adjoint(m, m.clone(), ks).shape == (0, 64, 64)

@@ -455,6 +455,17 @@ def main():
                      "avg_launch_ms": round(pass_ms, 5),
                      "stage_avg_ms": {k: round(v, 5) for k, v in stage_avg.items()},
                      "dominant_stage": dominant},
+        "reference_equivalent_flops": {
+            "definition": "F_ref = (C + K C) 10 N^2 log2 N flop per image: the reference's "
+                          "formulation (one spectrum FFT and K kernel FFTs per channel, "
+                          "5 N log2 N per 1-D transform), C = 1",
+            "flop_per_image_k120": (1 + 120) * 10 * N * N * math.log2(N),
+            "tflops_k120": round((1 + 120) * 10 * N * N * math.log2(N) * value / 1e12, 1),
+            "flop_per_image_k24": (1 + 24) * 10 * N * N * math.log2(N),
+            "tflops_k24": round((1 + 24) * 10 * N * N * math.log2(N) * B * args.steps / wall24 / 1e12, 1),
+            "fp32_vector_peak_tflops": 157.3,
+            "note": "exceeds the fp32 peak: the pruned, band-limited, paired-kernel formulation "
+                    "does far less arithmetic than the reference's"},
         "two_stream_batches": two,
         "socs_k24": {"images_per_s": round(B * args.steps / wall24, 2),
                      "kernels_per_pass": pk24.K,

@@ -27,6 +27,7 @@ host-only executor with simulated durations.
 from __future__ import annotations
 
 import collections
+import gc
 import time
 from dataclasses import dataclass, field
 
@@ -138,13 +139,17 @@ def pinned_masks(rids, sizes, make):
 
 class GpuExecutor:
     """Runs batches on `streams` HIP streams (default priority), one plan per
-    (size class, stream)."""
+    (size class, stream).  Each (size class, stream) also owns its input
+    buffers (float32 masks and byte codes for max_batch clips), allocated here
+    once: a stream runs one batch at a time, so launches allocate only their
+    output images."""
 
     def __init__(self, classes, streams: int = 4):
         from . import engine
         self.engine = engine
         self.streams = [torch.cuda.Stream() for _ in range(streams)]
         self.plans = {}
+        self.inputs = {}
         self.resident = 0
         for c in classes.values():
             ks = c.kernels
@@ -158,6 +163,10 @@ class GpuExecutor:
                                 band_c=min(bc, ks.wc))
                 self.plans[(c.n, si)] = p
                 self.resident += p.bytes
+                shape = (c.max_batch, c.n, c.n)
+                self.inputs[(c.n, si)] = (torch.empty(shape, dtype=torch.float32, device="cuda"),
+                                          torch.empty(shape, dtype=torch.uint8, device="cuda"))
+                self.resident += 5 * c.max_batch * c.n * c.n
 
     @property
     def slots(self) -> int:
@@ -169,12 +178,11 @@ class GpuExecutor:
         n = cls.n
         srcs = [torch.as_tensor(r.mask) for r in reqs]
         keep = [srcs]
+        fin, cin = self.inputs[(n, slot)]
         with torch.cuda.stream(s):
-            dev_in = torch.empty((b, n, n), dtype=torch.float32, device="cuda")
+            dev_in = fin[:b]
+            codes = cin[:b]
             coded = [i for i, r in enumerate(reqs) if r.coded]
-            if coded:
-                codes = torch.empty((b, n, n), dtype=torch.uint8, device="cuda")
-                keep.append(codes)
             for i, m in enumerate(srcs):      # pinned host memory: asynchronous copies on s
                 dst = codes[i] if reqs[i].coded else dev_in[i]
                 dst.copy_(m, non_blocking=m.is_pinned())
@@ -187,7 +195,6 @@ class GpuExecutor:
                                          plan=self.plans[(n, slot)])
             ev = torch.cuda.Event()
             ev.record(s)
-        keep.append(dev_in)
         return {"event": ev, "out": out, "keep": keep}
 
     def done(self, handle) -> bool:
@@ -294,18 +301,26 @@ class Server:
             if self.classes[r.n].bytes_for(r) > self.room:
                 raise ValueError(f"request {r.rid} does not fit in the HBM budget")
         clock = clock or time.perf_counter
-        t0 = clock()
-        i = 0
-        total = len(requests)
-        while len(self.completed) < total:
-            now = clock() - t0
-            while i < total and requests[i].arrival <= now:
-                self.waiting.append(requests[i])
-                i += 1
-            before = (len(self.completed), len(self.running))
-            self.step(now)
-            if (len(self.completed), len(self.running)) == before and idle_sleep:
-                time.sleep(idle_sleep)
+        # the cyclic garbage collector is paused while serving: a full
+        # collection over the run's records stalls the loop for milliseconds
+        gc_on = gc.isenabled()
+        gc.disable()
+        try:
+            t0 = clock()
+            i = 0
+            total = len(requests)
+            while len(self.completed) < total:
+                now = clock() - t0
+                while i < total and requests[i].arrival <= now:
+                    self.waiting.append(requests[i])
+                    i += 1
+                before = (len(self.completed), len(self.running))
+                self.step(now)
+                if (len(self.completed), len(self.running)) == before and idle_sleep:
+                    time.sleep(idle_sleep)
+        finally:
+            if gc_on:
+                gc.enable()
         return self.completed
 
 

@@ -121,6 +121,48 @@ __device__ __forceinline__ void rows_separate(float2 zk, float2 zm, float2* dst)
   dst[1] = make_float2(0.5f * (zk.y + zm.y), -0.5f * (zk.x - zm.x));    // (Z[k] - conj Z[-k]) / 2i
 }
 
+// Inverse transform (e^{+2 pi i k x / N}) of v, keeping X[t], X[t + T] and
+// X[N - T + t] of the group's thread t: stage 1 as GFFT::run_gen, the
+// exchange, then outputs 0, 1 and V - 1 of the stage-2 DFT only,
+// sum_j b_j e^{+2 pi i j k / V}.  Every thread of the block calls (barrier).
+template <class P>
+__device__ __forceinline__ void narrow_inverse(const float2 (&v)[P::V], float2* scr, int t,
+                                               const float2* __restrict__ twt, float2& o0,
+                                               float2& o1, float2& om) {
+  constexpr int TT = P::T, V = P::V, R1 = P::R1, U1 = P::U1;
+  static_assert(P::U2 == 1 && P::R2 == V && TT == R1, "radix-V second stage");
+#pragma unroll
+  for (int u = 0; u < U1; ++u) {
+    float2 b[R1];
+#pragma unroll
+    for (int j = 0; j < R1; ++j) b[j] = v[u + j * U1];
+    reg::DFT<R1, true>::run(b);
+    const int i = t + TT * u;
+#pragma unroll
+    for (int j = 0; j < R1; ++j) scr[P::xi(i * R1 + j)] = b[j];
+  }
+  __syncthreads();
+  float sx = 0.f, sy = 0.f, A = 0.f, B = 0.f, C = 0.f, D = 0.f;
+#pragma unroll
+  for (int j = 0; j < V; ++j) {
+    float2 b = scr[P::xi(t + j * R1)];
+    if (j > 0) {
+      const float2 w = twt[j * t];                  // exp(-2 pi i j t / N), used conjugated
+      b = reg::mul(b, make_float2(w.x, -w.y));
+    }
+    const float c = (float)reg::root_re(j, V), s = (float)(-reg::root_im(j, V));
+    sx += b.x;
+    sy += b.y;
+    A = fmaf(b.x, c, A);
+    B = fmaf(b.y, s, B);
+    C = fmaf(b.x, s, C);
+    D = fmaf(b.y, c, D);
+  }
+  o0 = make_float2(sx, sy);
+  o1 = make_float2(A - B, C + D);
+  om = make_float2(A + B, D - C);
+}
+
 template <class P>
 __global__ __launch_bounds__(kBlock) void k_fp_rows_real_narrow(const float* __restrict__ mask,
                                                                 int ny, int k0c, int wc,
@@ -152,38 +194,8 @@ __global__ __launch_bounds__(kBlock) void k_fp_rows_real_narrow(const float* __r
 #pragma unroll
     for (int m = 0; m < V; ++m) v[m] = nxt[m];
     if (it + 1 < kRowSets && set + 1 < sets) rows_load_pair<P>(nxt, src, 2 * ((set + 1) * G + g), t);
-    // stage 1 (as GFFT::run_gen), outputs to the exchange scratch
-#pragma unroll
-    for (int u = 0; u < U1; ++u) {
-      float2 b[R1];
-#pragma unroll
-      for (int j = 0; j < R1; ++j) b[j] = v[u + j * U1];
-      reg::DFT<R1, true>::run(b);
-      const int i = t + TT * u;
-#pragma unroll
-      for (int j = 0; j < R1; ++j) scr[P::xi(i * R1 + j)] = b[j];
-    }
-    __syncthreads();
-    // stage 2, outputs 0, 1 and V - 1 only: sum_j b_j e^{+2 pi i j k / V}
-    float sx = 0.f, sy = 0.f, A = 0.f, B = 0.f, C = 0.f, D = 0.f;
-#pragma unroll
-    for (int j = 0; j < V; ++j) {
-      float2 b = scr[P::xi(t + j * R1)];
-      if (j > 0) {
-        const float2 w = twt[j * t];              // exp(-2 pi i j t / N), used conjugated
-        b = reg::mul(b, make_float2(w.x, -w.y));
-      }
-      const float c = (float)reg::root_re(j, V), s = (float)(-reg::root_im(j, V));
-      sx += b.x;
-      sy += b.y;
-      A = fmaf(b.x, c, A);
-      B = fmaf(b.y, s, B);
-      C = fmaf(b.x, s, C);
-      D = fmaf(b.y, c, D);
-    }
-    const float2 o0 = make_float2(sx, sy);          // X[t]
-    const float2 o1 = make_float2(A - B, C + D);    // X[t + T]
-    const float2 om = make_float2(A + B, D - C);    // X[N - T + t]
+    float2 o0, o1, om;                              // X[t], X[t + T], X[N - T + t]
+    narrow_inverse<P>(v, scr, t, twt, o0, o1, om);
     const float2 p0 = make_float2(__shfl(o0.x, partner), __shfl(o0.y, partner));
     const float2 pm = make_float2(__shfl(om.x, partner), __shfl(om.y, partner));
     const int ja = t - k0c, jb = t - TT - k0c, jc = TT - k0c;
@@ -268,6 +280,34 @@ __global__ __launch_bounds__(kBlock) void k_fp_cols(const float2* __restrict__ T
     const int i = kappa - k0r;
     if (i >= 0 && i < wr) dst[(size_t)i * wc + j] = make_float2(v[m].x * scale, v[m].y * scale);
   }
+}
+
+// Column kernel for a window inside [-T, T] (the spectrum rows k0r ..
+// k0r + wr - 1): only outputs kappa = t, t - T and T are formed.
+template <class P>
+__global__ __launch_bounds__(kBlock) void k_fp_cols_narrow(const float2* __restrict__ T, int k0r,
+                                                           int wr, int wc, float scale,
+                                                           const float2* __restrict__ twt,
+                                                           float2* __restrict__ spec) {
+  constexpr int N = P::N, TT = P::T, V = P::V, G = groups_per_block<P>();
+  static_assert(kBlock % TT == 0, "whole groups");
+  extern __shared__ float2 lds[];
+  const int g = threadIdx.x / TT, t = threadIdx.x - g * TT;
+  const int bc = blockIdx.y;
+  const int j = blockIdx.x * G + g;
+  const bool col = j < wc;
+  const float2* src = T + ((size_t)bc * wc + (col ? j : 0)) * N;   // column j, contiguous rows
+  float2 v[V];
+#pragma unroll
+  for (int m = 0; m < V; ++m) v[m] = col ? src[t + TT * m] : make_float2(0.f, 0.f);
+  float2 o0, o1, om;
+  narrow_inverse<P>(v, lds + g * P::LDS, t, twt, o0, o1, om);
+  if (!col) return;
+  float2* dst = spec + (size_t)bc * wr * wc + j;
+  const int ia = t - k0r, ib = t - TT - k0r, ic = TT - k0r;
+  if (ia >= 0 && ia < wr) dst[(size_t)ia * wc] = make_float2(o0.x * scale, o0.y * scale);
+  if (ib >= 0 && ib < wr) dst[(size_t)ib * wc] = make_float2(om.x * scale, om.y * scale);
+  if (t == 0 && ic >= 0 && ic < wr) dst[(size_t)ic * wc] = make_float2(o1.x * scale, o1.y * scale);
 }
 
 // -------------------------------------------------------------- coherent
@@ -663,7 +703,9 @@ __global__ __launch_bounds__(kBandBlock) void k_fp_band(const float2* __restrict
 }
 
 // U[img][q][y] = sum_qr Ihat[img][q][qr] e^{-2 pi i qr y / ny}
-template <class P>
+// Q > 0: qrm < Q T, so only registers m < Q and m >= V - Q can hold band
+// values (as in k_fp_urows).
+template <class P, int Q>
 __global__ __launch_bounds__(kBlock) void k_fp_ucols(const float2* __restrict__ Ihat, int qc, int qrm,
                                                      const float2* __restrict__ twt,
                                                      float2* __restrict__ U) {
@@ -677,12 +719,14 @@ __global__ __launch_bounds__(kBlock) void k_fp_ucols(const float2* __restrict__ 
   const int nq = qc + 1;
   const bool on = live && q < nq;
   const float2* src = Ihat + ((size_t)img * nq + (on ? q : 0)) * (2 * qrm + 1);
+  static_assert(Q >= 0 && 2 * Q <= V, "band registers");
   float2 v[V];
 #pragma unroll
   for (int m = 0; m < V; ++m) {
     const int y = t + TT * m;
     const int qr = y <= qrm ? y : y - N;
-    v[m] = (on && qr >= -qrm && qr <= qrm) ? src[qr + qrm] : make_float2(0.f, 0.f);
+    const bool may = Q == 0 || m < Q || m >= V - Q;      // compile-time for Q > 0
+    v[m] = (may && on && qr >= -qrm && qr <= qrm) ? src[qr + qrm] : make_float2(0.f, 0.f);
   }
   P::template run_live_tab<false>(v, lds + g * P::LDS, t, twt, true);
   if (!on) return;
@@ -692,7 +736,10 @@ __global__ __launch_bounds__(kBlock) void k_fp_ucols(const float2* __restrict__ 
 }
 
 // Image rows from U (Hermitian extension over q), two rows per transform.
-template <class P>
+// Q > 0: the band has qc < Q T, so thread t's inputs x = t + T m can be
+// non-zero only for m < Q or m >= V - Q; the other registers are the constant
+// zero and the first-stage arithmetic on them folds away at compile time.
+template <class P, int Q>
 __global__ __launch_bounds__(kBlock) void k_fp_urows(const float2* __restrict__ U, int ny, int qc,
                                                      const float2* __restrict__ twt,
                                                      float* __restrict__ out) {
@@ -714,12 +761,14 @@ __global__ __launch_bounds__(kBlock) void k_fp_urows(const float2* __restrict__ 
     tile[q * TS + r] = y < ny ? U[((size_t)img * nq + q) * ny + y] : make_float2(0.f, 0.f);
   }
   __syncthreads();
+  static_assert(Q >= 0 && 2 * Q <= V, "band registers");
   float2 v[V];
 #pragma unroll
   for (int m = 0; m < V; ++m) {
     const int x = t + TT * m;
     float2 z = make_float2(0.f, 0.f);
-    if (live && (x <= qc || x >= N - qc)) {
+    const bool may = Q == 0 || m < Q || m >= V - Q;      // compile-time for Q > 0
+    if (may && live && (x <= qc || x >= N - qc)) {
       const bool neg = x > qc;
       const int q = neg ? N - x : x;
       float2 u0 = tile[q * TS + 2 * gg], u1 = tile[q * TS + 2 * gg + 1];
@@ -802,38 +851,55 @@ template <class P>
 static int run_cols(const FpArgs& a, hipStream_t st) {
   constexpr int G = groups_per_block<P>();
   const size_t lds = rows_lds<P>();
-  int rc = allow(k_fp_cols<P>, lds);
-  if (rc) return rc;
   dim3 grid((a.wc + G - 1) / G, a.batch * a.nchan);
   const float scale = 1.0f / ((float)a.ny * (float)a.nx);
-  hipLaunchKernelGGL(k_fp_cols<P>, grid, dim3(kBlock), lds, st, a.T, a.k0r, a.wr, a.wc, scale,
-                     a.tw_y, a.spec);
-  OI_LAUNCH_CHECK("fp_cols");
-  return OI_OK;
+  auto go = [&](auto kernel) -> int {
+    int rc = allow(kernel, lds);
+    if (rc) return rc;
+    hipLaunchKernelGGL(kernel, grid, dim3(kBlock), lds, st, a.T, a.k0r, a.wr, a.wc, scale, a.tw_y,
+                       a.spec);
+    OI_LAUNCH_CHECK("fp_cols");
+    return OI_OK;
+  };
+  if (a.k0r >= -P::T && a.k0r + a.wr - 1 <= P::T) return go(k_fp_cols_narrow<P>);
+  return go(k_fp_cols<P>);
 }
 
 template <class P>
 static int run_ucols(const FpArgs& a, hipStream_t st) {
   constexpr int G = groups_per_block<P>();
   const size_t lds = rows_lds<P>();
-  int rc = allow(k_fp_ucols<P>, lds);
-  if (rc) return rc;
   dim3 grid((a.qc + 1 + G - 1) / G, a.batch);
-  hipLaunchKernelGGL(k_fp_ucols<P>, grid, dim3(kBlock), lds, st, a.Ihat, a.qc, a.qr, a.tw_y, a.U);
-  OI_LAUNCH_CHECK("fp_ucols");
-  return OI_OK;
+  auto go = [&](auto kernel) -> int {
+    int rc = allow(kernel, lds);
+    if (rc) return rc;
+    hipLaunchKernelGGL(kernel, grid, dim3(kBlock), lds, st, a.Ihat, a.qc, a.qr, a.tw_y, a.U);
+    OI_LAUNCH_CHECK("fp_ucols");
+    return OI_OK;
+  };
+  const int q = (a.qr + 1 + P::T - 1) / P::T;
+  if (q <= 2 && 2 * 2 <= P::V) return go(k_fp_ucols<P, 2>);
+  if (q <= 3 && 2 * 3 <= P::V) return go(k_fp_ucols<P, 3>);
+  return go(k_fp_ucols<P, 0>);
 }
 
 template <class P>
 static int run_urows(const FpArgs& a, hipStream_t st) {
   constexpr int G = groups_per_block<P>();
   const size_t lds = rows_lds<P>() + (size_t)(a.qc + 1) * (2 * G + 1) * sizeof(float2);
-  int rc = allow(k_fp_urows<P>, lds);
-  if (rc) return rc;
   dim3 grid((a.ny + 2 * G - 1) / (2 * G), a.batch);
-  hipLaunchKernelGGL(k_fp_urows<P>, grid, dim3(kBlock), lds, st, a.U, a.ny, a.qc, a.tw_x, a.out);
-  OI_LAUNCH_CHECK("fp_urows");
-  return OI_OK;
+  auto go = [&](auto kernel) -> int {
+    int rc = allow(kernel, lds);
+    if (rc) return rc;
+    hipLaunchKernelGGL(kernel, grid, dim3(kBlock), lds, st, a.U, a.ny, a.qc, a.tw_x, a.out);
+    OI_LAUNCH_CHECK("fp_urows");
+    return OI_OK;
+  };
+  // registers that can hold band values: Q = ceil((qc + 1) / T) at each end
+  const int q = (a.qc + 1 + P::T - 1) / P::T;
+  if (q <= 2 && 2 * 2 <= P::V) return go(k_fp_urows<P, 2>);
+  if (q <= 3 && 2 * 3 <= P::V) return go(k_fp_urows<P, 3>);
+  return go(k_fp_urows<P, 0>);
 }
 
 // The one-pass kernel serves coarse sizes with a block size in the table when

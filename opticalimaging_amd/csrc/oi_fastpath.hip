@@ -794,8 +794,13 @@ __global__ __launch_bounds__(kBlock) void k_fp_urows(const float2* __restrict__ 
 }
 
 // ---------------------------------------------------------------- tables
-// Image sizes: N = R1 x R2.
-#define OI_IMAGE_PLANS(X) X(128, 8, 16) X(256, 16, 16) X(512, 16, 32) X(1024, 32, 32)
+// Image sizes: N = R1 x R2 (two stages), 2048 = 16 x 8 x 16 (three stages).
+using Img128 = reg::GFFT<128, 8, 16>;
+using Img256 = reg::GFFT<256, 16, 16>;
+using Img512 = reg::GFFT<512, 16, 32>;
+using Img1024 = reg::GFFT<1024, 32, 32>;
+using Img2048 = reg::GFFT3<2048, 16, 8, 16>;
+#define OI_IMAGE_PLANS(X) X(128, Img128) X(256, Img256) X(512, Img512) X(1024, Img1024) X(2048, Img2048)
 // Coarse sizes: L = R1 x R2 and the block size of the one-pass coherent kernel
 // (0: the coarse grid is too large for it; the 256-thread kernel runs).
 #define OI_COARSE_PLANS(X)                                                                      \
@@ -822,13 +827,19 @@ static int run_rows(const FpArgs& a, hipStream_t st) {
   constexpr int G = groups_per_block<P>();
   // transform scratch + output tile [wc][2G + 1]
   const size_t lds = rows_lds<P>() + (size_t)a.wc * (2 * G + 1) * sizeof(float2);
-  if (a.kind == OI_FIELD_REAL && a.k0c >= -P::T && a.k0c + a.wc - 1 <= P::T && a.ny % (2 * G) == 0) {
-    int rc = allow(k_fp_rows_real_narrow<P>, lds);
-    if (rc) return rc;
-    const int sets = (a.ny + 2 * G - 1) / (2 * G);
-    dim3 grid((sets + kRowSets - 1) / kRowSets, a.batch);
-    hipLaunchKernelGGL(k_fp_rows_real_narrow<P>, grid, dim3(kBlock), lds, st,
-                       static_cast<const float*>(a.field), a.ny, a.k0c, a.wc, a.tw_x, a.T);
+  bool narrow = false;
+  if constexpr (P::STAGES == 2)
+    narrow = a.kind == OI_FIELD_REAL && a.k0c >= -P::T && a.k0c + a.wc - 1 <= P::T &&
+             a.ny % (2 * G) == 0;
+  if (narrow) {
+    if constexpr (P::STAGES == 2) {
+      int rc = allow(k_fp_rows_real_narrow<P>, lds);
+      if (rc) return rc;
+      const int sets = (a.ny + 2 * G - 1) / (2 * G);
+      dim3 grid((sets + kRowSets - 1) / kRowSets, a.batch);
+      hipLaunchKernelGGL(k_fp_rows_real_narrow<P>, grid, dim3(kBlock), lds, st,
+                         static_cast<const float*>(a.field), a.ny, a.k0c, a.wc, a.tw_x, a.T);
+    }
   } else if (a.kind == OI_FIELD_REAL) {
     int rc = allow(k_fp_rows_real<P>, lds);
     if (rc) return rc;
@@ -861,7 +872,9 @@ static int run_cols(const FpArgs& a, hipStream_t st) {
     OI_LAUNCH_CHECK("fp_cols");
     return OI_OK;
   };
-  if (a.k0r >= -P::T && a.k0r + a.wr - 1 <= P::T) return go(k_fp_cols_narrow<P>);
+  if constexpr (P::STAGES == 2) {
+    if (a.k0r >= -P::T && a.k0r + a.wr - 1 <= P::T) return go(k_fp_cols_narrow<P>);
+  }
   return go(k_fp_cols<P>);
 }
 
@@ -1027,13 +1040,13 @@ int fp_interp(const FpArgs& a, int groups, hipStream_t st) {
 #undef X
   if (rc) return rc == OI_ERR_UNSUPPORTED ? fail(rc, "fast path: unsupported coarse size") : rc;
   rc = OI_ERR_UNSUPPORTED;
-#define X(NN, A, B) if (a.ny == NN) rc = fp::run_ucols<reg::GFFT<NN, A, B>>(a, st);
+#define X(NN, PT) if (a.ny == NN) rc = fp::run_ucols<fp::PT>(a, st);
   OI_IMAGE_PLANS(X)
 #undef X
   if (rc) return rc;
   if (a.mark) a.mark(a.mark_ctx, 4);
   rc = OI_ERR_UNSUPPORTED;
-#define X(NN, A, B) if (a.nx == NN) rc = fp::run_urows<reg::GFFT<NN, A, B>>(a, st);
+#define X(NN, PT) if (a.nx == NN) rc = fp::run_urows<fp::PT>(a, st);
   OI_IMAGE_PLANS(X)
 #undef X
   if (rc) return rc;
@@ -1041,7 +1054,7 @@ int fp_interp(const FpArgs& a, int groups, hipStream_t st) {
   return OI_OK;
 }
 bool fp_image_size_ok(int n) {
-#define X(NN, A, B) if (n == NN) return true;
+#define X(NN, PT) if (n == NN) return true;
   OI_IMAGE_PLANS(X)
 #undef X
   return false;
@@ -1087,13 +1100,13 @@ int fp_coherent_rows_fit(int L) {
 
 int fp_spectrum(const FpArgs& a, hipStream_t st) {
   int rc = OI_ERR_UNSUPPORTED;
-#define X(NN, A, B) if (a.nx == NN) rc = fp::run_rows<reg::GFFT<NN, A, B>>(a, st);
+#define X(NN, PT) if (a.nx == NN) rc = fp::run_rows<fp::PT>(a, st);
   OI_IMAGE_PLANS(X)
 #undef X
   if (rc) return rc == OI_ERR_UNSUPPORTED ? fail(rc, "fast path: unsupported nx") : rc;
   if (a.mark) a.mark(a.mark_ctx, 1);
   rc = OI_ERR_UNSUPPORTED;
-#define X(NN, A, B) if (a.ny == NN) rc = fp::run_cols<reg::GFFT<NN, A, B>>(a, st);
+#define X(NN, PT) if (a.ny == NN) rc = fp::run_cols<fp::PT>(a, st);
   OI_IMAGE_PLANS(X)
 #undef X
   if (rc) return rc == OI_ERR_UNSUPPORTED ? fail(rc, "fast path: unsupported ny") : rc;

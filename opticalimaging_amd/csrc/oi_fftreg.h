@@ -195,6 +195,7 @@ struct GFFT {
   static constexpr bool PAD = (N % 32) == 0;
   __device__ __forceinline__ static int xi(int i) { return PAD ? pad(i) : i; }
   static constexpr int LDS = R2 == 1 ? 0 : padded(N);  // scratch (float2) per group
+  static constexpr int STAGES = 2;
   static_assert(R2 == 1 ? R1 == N : (R1 * R2 == N && (V % R1) == 0 && (V % R2) == 0),
                 "GFFT: N = R1 R2 with the smaller radix dividing the larger");
 
@@ -328,6 +329,93 @@ struct GFFT {
     } else {
       __syncthreads();
     }
+  }
+};
+
+// GFFT3<N, R1, R2, R3>: an N = R1 R2 R3 point transform with three radix
+// stages and two LDS exchanges, for sizes whose two-stage form would hold too
+// many values per thread (2048 = 16 x 8 x 16: 16 values per thread, T = 128
+// threads per transform instead of 64 values per thread).  Same interface as
+// GFFT for the table-twiddle path: thread t holds x[t + T m] on entry and
+// X[t + T m] on exit.
+//
+// With n = i + M j (M = N / R1) and k = k1 + R1 k', and then i = a + R3 b,
+// k' = c + R2 d:
+//   stage 1 (registers): Y[i][k1] = sum_j x[i + M j] w_R1^{j k1}, stored at i R1 + k1;
+//   stage 2: Z[k1][a][c] = w_M^{a c} sum_b w_R2^{b c} w_N^{(a + R3 b) k1} Y[a + R3 b][k1]
+//            for p = k1 + R1 a, read at p + (N / R2) b, stored at a R1 R2 + k1 + R1 c;
+//   stage 3 (p3 = k1 + R1 c): X[p3 + (N / R3) d] = sum_a w_R3^{a d} Z[k1][a][c].
+// Every thread of the block calls (three workgroup barriers inside).
+template <int N_, int R1_, int R2_, int R3_>
+struct GFFT3 {
+  static constexpr int N = N_;
+  static constexpr int R1 = R1_, R2 = R2_, R3 = R3_;
+  static constexpr int V = R1 > R2 ? (R1 > R3 ? R1 : R3) : (R2 > R3 ? R2 : R3);
+  static constexpr int T = N / V;
+  static constexpr int U1 = V / R1, U2 = V / R2, U3 = V / R3;
+  static constexpr bool PAD = (N % 32) == 0;
+  static constexpr int LDS = padded(N);
+  static constexpr int STAGES = 3;
+  static_assert(R1 * R2 * R3 == N && V % R1 == 0 && V % R2 == 0 && V % R3 == 0,
+                "GFFT3: N = R1 R2 R3 with every radix dividing the largest");
+  __device__ __forceinline__ static int xi(int i) { return PAD ? pad(i) : i; }
+
+  // table[m] = exp(-2 pi i m / N), m in [0, N); INV uses it conjugated.
+  template <bool INV>
+  __device__ __forceinline__ static float2 tw(const float2* __restrict__ table, int m) {
+    const float2 w = table[m];
+    return INV ? make_float2(w.x, -w.y) : w;
+  }
+
+  template <bool INV>
+  __device__ __forceinline__ static void run_live_tab(float2 (&v)[V], float2* lds, int t,
+                                                      const float2* __restrict__ table, bool live) {
+#pragma unroll
+    for (int u = 0; u < U1; ++u) {
+      float2 b[R1];
+#pragma unroll
+      for (int j = 0; j < R1; ++j) b[j] = v[u + j * U1];
+      DFT<R1, INV>::run(b);
+      const int i = t + T * u;
+      if (live) {
+#pragma unroll
+        for (int k1 = 0; k1 < R1; ++k1) lds[xi(i * R1 + k1)] = b[k1];
+      }
+    }
+    __syncthreads();
+    float2 z[U2][R2];
+#pragma unroll
+    for (int u = 0; u < U2; ++u) {
+      const int p = t + T * u, k1 = p % R1, a = p / R1;
+#pragma unroll
+      for (int bb = 0; bb < R2; ++bb) {
+        const float2 x = lds[xi(p + (N / R2) * bb)];
+        z[u][bb] = mul(x, tw<INV>(table, (a + R3 * bb) * k1));
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < U2; ++u) {
+      const int p = t + T * u, k1 = p % R1, a = p / R1;
+      DFT<R2, INV>::run(z[u]);
+      if (live) {
+#pragma unroll
+        for (int c = 0; c < R2; ++c)
+          lds[xi(a * R1 * R2 + k1 + R1 * c)] = mul(z[u][c], tw<INV>(table, R1 * a * c));
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < U3; ++u) {
+      const int p3 = t + T * u;
+      float2 b[R3];
+#pragma unroll
+      for (int aa = 0; aa < R3; ++aa) b[aa] = lds[xi(p3 + (N / R3) * aa)];
+      DFT<R3, INV>::run(b);
+#pragma unroll
+      for (int d = 0; d < R3; ++d) v[u + U3 * d] = b[d];
+    }
+    __syncthreads();
   }
 };
 

@@ -99,8 +99,8 @@ def test_fast_abbe_1024_config3_optics(gpu):
 @pytest.mark.parametrize("n", [1024, 2048])
 def test_two_pass_large_coarse_grid(gpu, n):
     """Kernels of the NA 1.35 quasar optics on 4 nm pixels: windows of 109 / 217
-    orders need coarse grids of 256 / 512 (two-pass coherent sum); 2048^2 also
-    takes the generic spectrum and interpolation kernels."""
+    orders need coarse grids of 256 / 512 (two-pass coherent sum); 2048^2
+    images take the three-stage register transforms (16 x 8 x 16)."""
     from opticalimaging_amd import engine, workloads
     from opticalimaging_amd.DecomposeTCC_SOCS import socs_kernel_set
     src, proj, num = workloads.ilt_optics()
@@ -172,3 +172,27 @@ def test_empty_batch_is_a_no_op(gpu):
     m = torch.zeros((0, 64, 64), device="cuda")
     assert engine.socs_image(m, ks).shape == (0, 64, 64)
     assert engine.socs_adjoint(m, m.clone(), ks).shape == (0, 64, 64)
+
+
+def test_fast_2048_one_pass(gpu):
+    """2048^2 image (2 nm pixels, 4096 nm period) with a 65-order window: the
+    one-pass coherent kernel with the three-stage 2048-point row, column and
+    interpolation transforms, against the float64 oracle."""
+    from opticalimaging_amd import engine
+    n, period = 2048, 4096.0
+    src, proj, num = annular(), projector(), numerics(order=8)
+    Gs, V, f, g = O.shifted_pupil_matrix(src, [period, period], proj, 0.0, num)
+    kern = O.decompose_lowrank(Gs, V, 1.0, [len(g), len(f)], [n, n], num)
+    ks = engine.KernelSet.from_dense(torch.as_tensor(kern.astype(np.complex64)))
+    plan = engine.plan_for(ks, 1, 2)
+    assert plan.info()["path"] == 1, plan.info()
+    masks = _masks(n, 2, seed=11)
+    img = engine.socs_image(torch.as_tensor(masks).cuda(), ks, plan=plan, pack_real=False).cpu().numpy()
+    for b in range(2):
+        f3 = np.zeros((n, n, 3))
+        f3[..., 0] = masks[b]
+        ref = O.socs_image(f3, kern, period, period, proj.NA, src.Wavelength)
+        err = rel_err(img[b], ref)
+        print(f"fast 2048^2 coarse {plan.info()['lr']} window {ks.wr} img {b}: {err:.2e}")
+        assert err < 1e-5
+

@@ -61,7 +61,7 @@ def parse():
     ap.add_argument("--cpu-sample", type=int, default=6,
                     help="masks in the CPU K=24 SOCS sample (a third of them for CPU Abbe)")
     ap.add_argument("--no-extras", action="store_true",
-                    help="skip accuracy, plain-Abbe, TCC and CPU legs (profiling runs)")
+                    help="skip accuracy, plain-Abbe, TCC, two-stream and CPU legs (profiling runs)")
     ap.add_argument("--serving-requests", type=int, default=10000,
                     help="requests in the serving leg (0 skips it)")
     ap.add_argument("--serving-rate", type=float, default=3000.0,
@@ -421,8 +421,10 @@ def main():
         return {"images_per_s": round(B * steps / dt, 2), "ms_per_batch": round(dt / steps * 1e3, 4),
                 "images_equal_single_stream": same}
 
-    two = two_streams(ks_full, 2 * (args.steps // 2))
-    log(f"two streams, consecutive batches: {json.dumps(two)}")
+    two = None
+    if not args.no_extras:
+        two = two_streams(ks_full, 2 * (args.steps // 2))
+        log(f"two streams, consecutive batches: {json.dumps(two)}")
 
     line = {
         "metric": METRIC,

@@ -100,6 +100,8 @@ class KernelSet:
 
 
 PAIR_MAX_K = 512        # larger sets are imaged as they are (the 2K x 2K eigenproblem grows)
+PAIR_MIN_BATCH = 4      # pairing (a 2K x 2K eigenproblem) is first built for a batch this
+                        # large; a one-off image of a few masks keeps the original kernels
 
 
 def real_mask_kernels(ks: KernelSet, rel_tol: float = 1e-10) -> KernelSet:
@@ -270,7 +272,8 @@ def socs_image(masks: torch.Tensor, ks: KernelSet, kind: int = nat.OI_FIELD_REAL
     device.  Returns float32 [B, ny, nx] in the reference's output layout
     (after its final fftshift).  Real masks are imaged with
     ks.for_real_masks() (the same image with up to half the kernels) unless
-    pack_real is False."""
+    pack_real is False; that set is built on the first call with at least
+    PAIR_MIN_BATCH masks (or by calling ks.for_real_masks()) and reused after."""
     B = int(masks.shape[0])
     ny, nx = int(masks.shape[1]), int(masks.shape[2])
     if (ny, nx) != (ks.ny, ks.nx):
@@ -278,7 +281,7 @@ def socs_image(masks: torch.Tensor, ks: KernelSet, kind: int = nat.OI_FIELD_REAL
     if B == 0:
         return out if out is not None else torch.empty((0, ny, nx), dtype=torch.float32,
                                                        device=masks.device)
-    if kind == nat.OI_FIELD_REAL and pack_real:
+    if kind == nat.OI_FIELD_REAL and pack_real and (ks._real is not None or B >= PAIR_MIN_BATCH):
         ks = ks.for_real_masks()
     chans = list(chans)
     nchan = len(chans)
@@ -316,7 +319,7 @@ def socs_adjoint(masks: torch.Tensor, grad_image: torch.Tensor, ks: KernelSet,
                                                        device=masks.device)
     masks = masks.contiguous()
     grad_image = grad_image.contiguous()
-    if pack_real:
+    if pack_real and (ks._real is not None or B >= PAIR_MIN_BATCH):
         ks = ks.for_real_masks()
     if plan is None:
         plan = plan_for(ks, 1, B)

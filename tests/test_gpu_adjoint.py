@@ -101,6 +101,7 @@ def test_adjoint_dot_product_1024(gpu):
     gen = torch.Generator(device="cuda").manual_seed(5)
     d = 0.5 * torch.randn(h.shape, device="cuda", generator=gen)
     g = torch.randn(h.shape, device="cuda", generator=gen)
+    ks.for_real_masks()                 # the paired set, used by the pack_real=True calls
     for pack in (True, False):
         grad = engine.socs_adjoint(h, g, ks, pack_real=pack)
         Ip = engine.socs_image(h + d, ks, pack_real=pack)

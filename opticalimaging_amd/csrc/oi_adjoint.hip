@@ -30,6 +30,8 @@
 //   k_adj_gcols / k_adj_grows  window -> image grid, real part, two rows per
 //                 complex transform (Hermitian packing)
 #include "oi_adjoint.h"
+#include "oi_fastpath.h"
+#include "oi_fftreg.h"
 
 namespace oi {
 
@@ -156,6 +158,194 @@ __global__ __launch_bounds__(kThreads) void k_adj_cwin(const float2* __restrict_
   }
 }
 
+// ---------------------------------------------------- register transforms
+// The three coarse-grid stages above with compile-time transform sizes (the
+// coarse sizes of the fast imaging path): a group of P::T lanes holds one
+// coarse row or window column, x[t + T m] in registers, one LDS exchange per
+// transform.  Same arithmetic as the LDS kernels; window indices fold onto
+// the coarse grid without collisions (wr <= lr, wc <= lc).
+constexpr int kRegBlock = 256;
+
+template <class P>
+__global__ __launch_bounds__(kRegBlock) void k_adj_cfield_reg(const float2* __restrict__ spec,
+                                                              const float2* __restrict__ kern,
+                                                              int nk, int k0r, int wr, int wc,
+                                                              const float2* __restrict__ twt,
+                                                              float2* __restrict__ Ec) {
+  constexpr int L = P::N, T = P::T, V = P::V, G = kRegBlock / T;
+  extern __shared__ float2 lds[];
+  const int g = threadIdx.x / T, t = threadIdx.x - g * T;
+  const int k = blockIdx.y, b = blockIdx.z;
+  const int kc = blockIdx.x * G + g;
+  const bool on = g < G && kc < wc;
+  const float2* sb = spec + (size_t)b * wr * wc + (on ? kc : 0);
+  const float2* kb = kern ? kern + (size_t)k * wr * wc + (on ? kc : 0) : nullptr;
+  float2 v[V];
+#pragma unroll
+  for (int m = 0; m < V; ++m) {
+    const int ir = pmod(t + T * m - k0r, L);
+    float2 a = make_float2(0.f, 0.f);
+    if (on && ir < wr) {
+      a = sb[(size_t)ir * wc];
+      if (kb) a = cmul(a, kb[(size_t)ir * wc]);
+    }
+    v[m] = a;
+  }
+  P::template run_live_tab<false>(v, lds + (g < G ? g : 0) * P::LDS, t, twt, g < G);
+  if (!on) return;
+  float2* out = Ec + ((size_t)b * nk + k) * L * wc + kc;
+#pragma unroll
+  for (int m = 0; m < V; ++m) out[(size_t)(t + T * m) * wc] = v[m];
+}
+
+template <class P>
+__global__ __launch_bounds__(kRegBlock) void k_adj_rfield_reg(float2* __restrict__ Ec, int nk,
+                                                              int k0c, int wc, int lr,
+                                                              const float2* __restrict__ twt,
+                                                              const float* __restrict__ gamma_in,
+                                                              float* __restrict__ gamma_out) {
+  constexpr int L = P::N, T = P::T, V = P::V, G = kRegBlock / T;
+  extern __shared__ float2 lds[];
+  const int g = threadIdx.x / T, t = threadIdx.x - g * T;
+  const int k = blockIdx.y, b = blockIdx.z;
+  const int r = blockIdx.x * G + g;
+  const bool on = g < G && r < lr;
+  float2* scr = lds + (g < G ? g : 0) * P::LDS;
+  float2* row = Ec + (((size_t)b * nk + k) * lr + (on ? r : 0)) * wc;
+  float2 v[V];
+#pragma unroll
+  for (int m = 0; m < V; ++m) {
+    const int kc = pmod(t + T * m - k0c, L);
+    v[m] = (on && kc < wc) ? row[kc] : make_float2(0.f, 0.f);
+  }
+  P::template run_live_tab<false>(v, scr, t, twt, g < G);
+  if (!gamma_in) {
+    if (on) {
+      float* go = gamma_out + ((size_t)b * lr + r) * L;
+#pragma unroll
+      for (int m = 0; m < V; ++m) go[t + T * m] = v[m].x;
+    }
+    return;
+  }
+  const float* gi = gamma_in + ((size_t)b * lr + (on ? r : 0)) * L;
+#pragma unroll
+  for (int m = 0; m < V; ++m) {
+    const float gm = on ? gi[t + T * m] : 0.f;
+    v[m] = make_float2(gm * v[m].x, -gm * v[m].y);
+  }
+  P::template run_live_tab<false>(v, scr, t, twt, g < G);
+  if (!on) return;
+#pragma unroll
+  for (int m = 0; m < V; ++m) {
+    const int kc = pmod(t + T * m - k0c, L);
+    if (kc < wc) row[kc] = v[m];
+  }
+}
+
+template <class P>
+__global__ __launch_bounds__(kRegBlock) void k_adj_cwin_reg(const float2* __restrict__ Pc,
+                                                            const float2* __restrict__ kern, int nk,
+                                                            int k0r, int wr, int wc,
+                                                            const float2* __restrict__ twt,
+                                                            float2* __restrict__ Ak) {
+  constexpr int L = P::N, T = P::T, V = P::V, G = kRegBlock / T;
+  extern __shared__ float2 lds[];
+  const int g = threadIdx.x / T, t = threadIdx.x - g * T;
+  const int k = blockIdx.y, b = blockIdx.z;
+  const int kc = blockIdx.x * G + g;
+  const bool on = g < G && kc < wc;
+  const float2* pb = Pc + ((size_t)b * nk + k) * L * wc + (on ? kc : 0);
+  float2 v[V];
+#pragma unroll
+  for (int m = 0; m < V; ++m) v[m] = on ? pb[(size_t)(t + T * m) * wc] : make_float2(0.f, 0.f);
+  P::template run_live_tab<false>(v, lds + (g < G ? g : 0) * P::LDS, t, twt, g < G);
+  if (!on) return;
+  const float2* kb = kern + (size_t)k * wr * wc + kc;
+  float2* out = Ak + ((size_t)b * nk + k) * wr * wc + kc;
+#pragma unroll
+  for (int m = 0; m < V; ++m) {
+    const int kr = pmod(t + T * m - k0r, L);
+    if (kr < wr) out[(size_t)kr * wc] = cmul(kb[(size_t)kr * wc], v[m]);
+  }
+}
+
+// Coarse sizes with register transforms: L = R1 x R2 (the smaller radix
+// dividing the larger), as the fast imaging path's coarse grids.
+#define OI_ADJ_PLANS(X)                                                                        \
+  X(16, 4, 4) X(25, 5, 5) X(32, 4, 8) X(36, 6, 6) X(48, 4, 12) X(64, 8, 8) X(75, 5, 15)       \
+  X(81, 9, 9) X(100, 10, 10) X(128, 8, 16) X(144, 12, 12) X(256, 16, 16) X(512, 16, 32)
+
+static bool adj_reg_size(int L) {
+#define X(NN, A, B) if (L == NN) return true;
+  OI_ADJ_PLANS(X)
+#undef X
+  return false;
+}
+
+template <class P>
+static size_t adj_reg_lds() {
+  return (size_t)(kRegBlock / P::T) * P::LDS * sizeof(float2);
+}
+
+// Column stage (kern == nullptr: gamma's spectrum) with register transforms.
+static int adj_cfield_reg(int L, dim3 grid_y_z, const float2* spec, const float2* kern, int nk,
+                          int k0r, int wr, int wc, const float2* twt, float2* Ec, hipStream_t st) {
+#define X(NN, A, B)                                                                            \
+  if (L == NN) {                                                                               \
+    using P = reg::GFFT<NN, A, B>;                                                             \
+    constexpr int G = kRegBlock / P::T;                                                        \
+    int rc = allow_lds_adj(k_adj_cfield_reg<P>, adj_reg_lds<P>());                             \
+    if (rc) return rc;                                                                         \
+    dim3 grid((wc + G - 1) / G, grid_y_z.y, grid_y_z.z);                                       \
+    hipLaunchKernelGGL(k_adj_cfield_reg<P>, grid, dim3(kRegBlock), adj_reg_lds<P>(), st, spec, \
+                       kern, nk, k0r, wr, wc, twt, Ec);                                        \
+    OI_LAUNCH_CHECK("adj_cfield_reg");                                                         \
+    return OI_OK;                                                                              \
+  }
+  OI_ADJ_PLANS(X)
+#undef X
+  return fail(OI_ERR_UNSUPPORTED, "adjoint: no register transform for this coarse size");
+}
+
+static int adj_rfield_reg(int L, dim3 grid_y_z, float2* Ec, int nk, int k0c, int wc, int lr,
+                          const float2* twt, const float* gamma_in, float* gamma_out,
+                          hipStream_t st) {
+#define X(NN, A, B)                                                                            \
+  if (L == NN) {                                                                               \
+    using P = reg::GFFT<NN, A, B>;                                                             \
+    constexpr int G = kRegBlock / P::T;                                                        \
+    int rc = allow_lds_adj(k_adj_rfield_reg<P>, adj_reg_lds<P>());                             \
+    if (rc) return rc;                                                                         \
+    dim3 grid((lr + G - 1) / G, grid_y_z.y, grid_y_z.z);                                       \
+    hipLaunchKernelGGL(k_adj_rfield_reg<P>, grid, dim3(kRegBlock), adj_reg_lds<P>(), st, Ec,   \
+                       nk, k0c, wc, lr, twt, gamma_in, gamma_out);                             \
+    OI_LAUNCH_CHECK("adj_rfield_reg");                                                         \
+    return OI_OK;                                                                              \
+  }
+  OI_ADJ_PLANS(X)
+#undef X
+  return fail(OI_ERR_UNSUPPORTED, "adjoint: no register transform for this coarse size");
+}
+
+static int adj_cwin_reg(int L, dim3 grid_y_z, const float2* Pc, const float2* kern, int nk,
+                        int k0r, int wr, int wc, const float2* twt, float2* Ak, hipStream_t st) {
+#define X(NN, A, B)                                                                            \
+  if (L == NN) {                                                                               \
+    using P = reg::GFFT<NN, A, B>;                                                             \
+    constexpr int G = kRegBlock / P::T;                                                        \
+    int rc = allow_lds_adj(k_adj_cwin_reg<P>, adj_reg_lds<P>());                               \
+    if (rc) return rc;                                                                         \
+    dim3 grid((wc + G - 1) / G, grid_y_z.y, grid_y_z.z);                                       \
+    hipLaunchKernelGGL(k_adj_cwin_reg<P>, grid, dim3(kRegBlock), adj_reg_lds<P>(), st, Pc,     \
+                       kern, nk, k0r, wr, wc, twt, Ak);                                        \
+    OI_LAUNCH_CHECK("adj_cwin_reg");                                                           \
+    return OI_OK;                                                                              \
+  }
+  OI_ADJ_PLANS(X)
+#undef X
+  return fail(OI_ERR_UNSUPPORTED, "adjoint: no register transform for this coarse size");
+}
+
 // Q[b][i] = sum_k Ak[b][k][i], i over the window.
 __global__ void k_adj_ksum(const float2* __restrict__ Ak, int nk, int per, int batch,
                            float2* __restrict__ Q) {
@@ -273,6 +463,10 @@ int launch_adjoint(const AdjGeom& g, const float* gI, const float2* spec, int ba
   float2* Q = (float2*)carve(w, B * g.wr * g.wc * sizeof(float2));
   float2* Z = (float2*)carve(w, B * g.wc * g.ny * sizeof(float2));
 
+  // register transforms on the coarse grid when both sizes are tabled and
+  // the window (and gamma's band) fold onto it without collisions
+  const bool reg = !g.direct && adj_reg_size(g.lr) && adj_reg_size(g.lc) && g.wr <= g.lr &&
+                   g.wc <= g.lc && 2 * g.qr + 1 <= g.lr && 2 * g.qc + 1 <= g.lc;
   const int ncol_r = lds_items(g.lr, 64);
   const int nrow_c = lds_items(g.lc, 16);
   const size_t lds_r = 2ull * ncol_r * g.lr * sizeof(float2);
@@ -285,32 +479,77 @@ int launch_adjoint(const AdjGeom& g, const float* gI, const float2* spec, int ba
   if (!g.direct) {
     // gamma: the cotangent's spectrum on [-q, q]^2 taken to the coarse grid
     const int chan0 = 0;
-    rc = launch_spec_rows(gI, OI_FIELD_REAL, 1, &chan0, 1, batch, g.ny, g.nx, g.fx, -g.qc, wgc, Tg, st);
-    if (rc) return rc;
-    rc = launch_spec_cols(Tg, batch, g.ny, g.nx, g.fy, -g.qr, wgr, wgc, specg, st);
-    if (rc) return rc;
-    dim3 g1((wgc + ncol_r - 1) / ncol_r, 1, batch);
-    hipLaunchKernelGGL(k_adj_cfield, g1, dim3(kThreads), lds_r, st, specg, nullptr, 1, -g.qr, wgr,
-                       wgc, g.lr, ncol_r, g.flr, Ecg);
-    OI_LAUNCH_CHECK("adj_cfield(gamma)");
-    dim3 g2((g.lr + nrow_c - 1) / nrow_c, 1, batch);
-    hipLaunchKernelGGL(k_adj_rfield, g2, dim3(kThreads), lds_c, st, Ecg, 1, -g.qc, wgc, g.lr, g.lc,
-                       nrow_c, g.flc, nullptr, gamma);
-    OI_LAUNCH_CHECK("adj_rfield(gamma)");
+    if (fp_image_size_ok(g.ny) && fp_image_size_ok(g.nx)) {
+      // register transforms of the image sizes (same spectrum, scale and layout)
+      FpArgs a{};
+      a.field = gI;
+      a.kind = OI_FIELD_REAL;
+      a.cin = 1;
+      a.nchan = 1;
+      a.chans.c[0] = 0;
+      a.batch = batch;
+      a.ny = g.ny;
+      a.nx = g.nx;
+      a.k0r = -g.qr;
+      a.wr = wgr;
+      a.k0c = -g.qc;
+      a.wc = wgc;
+      a.tw_x = g.fx.tw;
+      a.tw_y = g.fy.tw;
+      a.T = Tg;
+      a.spec = specg;
+      rc = fp_spectrum(a, st);
+      if (rc) return rc;
+    } else {
+      rc = launch_spec_rows(gI, OI_FIELD_REAL, 1, &chan0, 1, batch, g.ny, g.nx, g.fx, -g.qc, wgc,
+                            Tg, st);
+      if (rc) return rc;
+      rc = launch_spec_cols(Tg, batch, g.ny, g.nx, g.fy, -g.qr, wgr, wgc, specg, st);
+      if (rc) return rc;
+    }
+    if (reg) {
+      if ((rc = adj_cfield_reg(g.lr, dim3(1, 1, batch), specg, nullptr, 1, -g.qr, wgr, wgc,
+                               g.flr.tw, Ecg, st)))
+        return rc;
+      if ((rc = adj_rfield_reg(g.lc, dim3(1, 1, batch), Ecg, 1, -g.qc, wgc, g.lr, g.flc.tw,
+                               nullptr, gamma, st)))
+        return rc;
+    } else {
+      dim3 g1((wgc + ncol_r - 1) / ncol_r, 1, batch);
+      hipLaunchKernelGGL(k_adj_cfield, g1, dim3(kThreads), lds_r, st, specg, nullptr, 1, -g.qr,
+                         wgr, wgc, g.lr, ncol_r, g.flr, Ecg);
+      OI_LAUNCH_CHECK("adj_cfield(gamma)");
+      dim3 g2((g.lr + nrow_c - 1) / nrow_c, 1, batch);
+      hipLaunchKernelGGL(k_adj_rfield, g2, dim3(kThreads), lds_c, st, Ecg, 1, -g.qc, wgc, g.lr,
+                         g.lc, nrow_c, g.flc, nullptr, gamma);
+      OI_LAUNCH_CHECK("adj_rfield(gamma)");
+    }
   }
   const float* gam = g.direct ? gI : gamma;
 
-  dim3 g3((g.wc + ncol_r - 1) / ncol_r, nkern, batch);
-  hipLaunchKernelGGL(k_adj_cfield, g3, dim3(kThreads), lds_r, st, spec, kern, nkern, g.k0r, g.wr,
-                     g.wc, g.lr, ncol_r, g.flr, Ec);
-  OI_LAUNCH_CHECK("adj_cfield");
-  dim3 g4((g.lr + nrow_c - 1) / nrow_c, nkern, batch);
-  hipLaunchKernelGGL(k_adj_rfield, g4, dim3(kThreads), lds_c, st, Ec, nkern, g.k0c, g.wc, g.lr,
-                     g.lc, nrow_c, g.flc, gam, nullptr);
-  OI_LAUNCH_CHECK("adj_rfield");
-  hipLaunchKernelGGL(k_adj_cwin, g3, dim3(kThreads), lds_r, st, Ec, kern, nkern, g.k0r, g.wr, g.wc,
-                     g.lr, ncol_r, g.flr, Ak);
-  OI_LAUNCH_CHECK("adj_cwin");
+  if (reg) {
+    if ((rc = adj_cfield_reg(g.lr, dim3(1, nkern, batch), spec, kern, nkern, g.k0r, g.wr, g.wc,
+                             g.flr.tw, Ec, st)))
+      return rc;
+    if ((rc = adj_rfield_reg(g.lc, dim3(1, nkern, batch), Ec, nkern, g.k0c, g.wc, g.lr, g.flc.tw,
+                             gam, nullptr, st)))
+      return rc;
+    if ((rc = adj_cwin_reg(g.lr, dim3(1, nkern, batch), Ec, kern, nkern, g.k0r, g.wr, g.wc,
+                           g.flr.tw, Ak, st)))
+      return rc;
+  } else {
+    dim3 g3((g.wc + ncol_r - 1) / ncol_r, nkern, batch);
+    hipLaunchKernelGGL(k_adj_cfield, g3, dim3(kThreads), lds_r, st, spec, kern, nkern, g.k0r, g.wr,
+                       g.wc, g.lr, ncol_r, g.flr, Ec);
+    OI_LAUNCH_CHECK("adj_cfield");
+    dim3 g4((g.lr + nrow_c - 1) / nrow_c, nkern, batch);
+    hipLaunchKernelGGL(k_adj_rfield, g4, dim3(kThreads), lds_c, st, Ec, nkern, g.k0c, g.wc, g.lr,
+                       g.lc, nrow_c, g.flc, gam, nullptr);
+    OI_LAUNCH_CHECK("adj_rfield");
+    hipLaunchKernelGGL(k_adj_cwin, g3, dim3(kThreads), lds_r, st, Ec, kern, nkern, g.k0r, g.wr,
+                       g.wc, g.lr, ncol_r, g.flr, Ak);
+    OI_LAUNCH_CHECK("adj_cwin");
+  }
   const int per = g.wr * g.wc;
   const int nblk = (int)((B * per + kThreads - 1) / kThreads);
   hipLaunchKernelGGL(k_adj_ksum, dim3(nblk < 4096 ? nblk : 4096), dim3(kThreads), 0, st, Ak, nkern,

@@ -70,19 +70,35 @@ def ilt_loss(theta: torch.Tensor, target: torch.Tensor, kernels) -> torch.Tensor
 
 
 def ilt(theta: torch.Tensor, target: torch.Tensor, kernels, steps: int = 50, lr: float = 0.5):
-    """Normalised gradient descent on theta (each image's step is lr times its
-    gradient divided by the gradient's RMS); returns (theta, per-step losses
-    tensor on the device).  The loop stays on the device (no host
+    """Normalised gradient descent on theta [B, Ny, Nx] (each image's step is
+    lr times its gradient divided by the gradient's RMS); returns (theta,
+    per-step losses tensor on the device).
+
+    The gradient is the one ``ilt_loss`` gives through autograd, written out
+    so that each step is a handful of device passes: m = sigmoid(theta),
+    I = SOCS(m), d = I - T, loss = d.d, dL/dm = adjoint(m, 2 d),
+    dL/dtheta = dL/dm m (1 - m).  The loop stays on the device (no host
     synchronisation per step)."""
     ks = kernel_set(kernels)
-    theta = theta.detach().clone()
+    theta = theta.detach().to(torch.float32).contiguous().clone()
+    if theta.dim() != 3:
+        raise ValueError("theta must be (B, Ny, Nx)")
+    target = target.detach().to(device=theta.device, dtype=torch.float32).contiguous()
+    B = int(theta.shape[0])
+    n = float(theta[0].numel())
+    plan = engine.plan_for(ks, 1, B)
     losses = torch.empty(steps, dtype=torch.float32, device=theta.device)
-    with torch.enable_grad():
-        for i in range(steps):
-            t = theta.requires_grad_(True)
-            loss = ilt_loss(t, target, ks)
-            (g,) = torch.autograd.grad(loss, t)
-            losses[i] = loss.detach()
-            rms = g.pow(2).mean(dim=(-2, -1), keepdim=True).sqrt() + 1e-30
-            theta = t.detach() - lr * g / rms
+    img = torch.empty_like(theta)
+    d = torch.empty_like(theta)
+    for i in range(steps):
+        m = torch.sigmoid(theta)
+        engine.socs_image(m, ks, out=img, plan=plan)
+        torch.sub(img, target, out=d)
+        flat = d.view(-1)
+        losses[i] = torch.dot(flat, flat)
+        d.mul_(2.0)
+        g = engine.socs_adjoint(m, d, ks, plan=plan)
+        g.mul_(m).mul_(1.0 - m)
+        rms = torch.linalg.vector_norm(g.view(B, -1), dim=1) / n ** 0.5 + 1e-30
+        theta.addcdiv_(g, rms.view(B, 1, 1), value=-lr)
     return theta, losses

@@ -91,7 +91,9 @@ typedef struct oi_geometry {
   int rows;
   int direct;
   int qr, qc;
-  int path;       /* 1: compile-time-size kernels, 0: generic kernels      */
+  int path;       /* 0: generic kernels; 1: compile-time-size kernels, one
+                     pass over the coarse grid; 2: compile-time-size kernels,
+                     two-pass coherent sum (coarse 256 or 512)               */
 } oi_geometry;
 
 typedef struct oi_plan oi_plan;

@@ -191,7 +191,13 @@ extern "C" int oi_plan_create(oi_geometry* geo, oi_plan** out) {
         fp_two_pass_lds(L, g.wc) <= kLdsLimit) {
       g.path = 2;
       g.lr = g.lc = lr = lc = L;
-      if (g.max_groups <= 0) g.max_groups = 1;
+      if (g.max_groups <= 0) {
+        // kernel groups of the row pass: about 1024 blocks (16 coarse rows
+        // each) over the batch, at most 8 partial sums per image
+        const int blocks = (L / 16) * g.max_batch;
+        int gr = (1024 + blocks - 1) / blocks;
+        g.max_groups = gr < 1 ? 1 : (gr > 8 ? 8 : gr);
+      }
     }
   }
   // Coarse rows per workgroup: largest of 16, 8, 4, 2, 1 dividing lr whose
@@ -396,17 +402,18 @@ extern "C" int oi_coherent_image(oi_plan* p, const void* spec, int batch, const 
     c.wr = g.wr;
     c.k0c = g.k0c;
     c.wc = g.wc;
-    c.groups = 1;
+    const bool fast_io = fp_image_size_ok(g.ny) && fp_image_size_ok(g.nx);
+    const int nkc = nkern * g.nchan;
+    c.groups = fast_io ? (groups < nkc ? groups : nkc) : 1;
     c.qc = g.qc;
     c.tw_l = p->flr.tw;
     c.tw_m = p->flr.tw;
     c.R = p->R;
-    const bool fast_io = fp_image_size_ok(g.ny) && fp_image_size_ok(g.nx);
     int rc = fp_two_pass(a, c, p->Bbuf, fast_io ? 0 : 1, st);
     if (rc) return rc;
     mark(p, 3, st);
     if (fast_io) {
-      rc = fp_interp(a, 1, st);
+      rc = fp_interp(a, c.groups, st);
     } else {
       InterpArgs ia{};
       ia.R = p->R;

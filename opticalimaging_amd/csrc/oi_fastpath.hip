@@ -595,6 +595,9 @@ __global__ __launch_bounds__(kBlock) void k_fp_bcols(const float2* __restrict__ 
   for (int m = 0; m < VL; ++m) dst[t + TL * m] = v[m];
 }
 
+// Blocks (row group, img, grp) sum the (kernel, channel) pairs kc = grp,
+// grp + groups, ... into their own partial R[img][grp] (groups = 1 for the
+// generic layout).
 template <class PL>
 __global__ __launch_bounds__(kBlock) void k_fp_brows(const float2* __restrict__ B, int nkc, int k0c,
                                                      int wc, int qc, int generic_layout,
@@ -607,7 +610,7 @@ __global__ __launch_bounds__(kBlock) void k_fp_brows(const float2* __restrict__ 
   float2* tile = lds;                      // [wc][TS]
   float2* scr = lds + (size_t)wc * TS;     // transform scratch
   const int g = threadIdx.x / TL, t = threadIdx.x - g * TL;
-  const int img = blockIdx.y;
+  const int img = blockIdx.y, grp = blockIdx.z, groups = gridDim.z;
   const int r0 = blockIdx.x * G;
   const int row = r0 + g;
   const bool live = g < G && row < L;
@@ -622,7 +625,7 @@ __global__ __launch_bounds__(kBlock) void k_fp_brows(const float2* __restrict__ 
   float acc[VL];
 #pragma unroll
   for (int m = 0; m < VL; ++m) acc[m] = 0.f;
-  for (int kc = 0; kc < nkc; ++kc) {
+  for (int kc = grp; kc < nkc; kc += groups) {
     const float2* src = B + ((size_t)img * nkc + kc) * wc * L + r0;
     __syncthreads();
     for (int e = threadIdx.x; e < wc * G; e += kBlock) {
@@ -649,7 +652,7 @@ __global__ __launch_bounds__(kBlock) void k_fp_brows(const float2* __restrict__ 
     const int x = t + TL * m;
     if (x > qc) continue;
     if (generic_layout) R[((size_t)img * L + row) * nq + x] = v[m];   // [img][t][q] (one group)
-    else R[((size_t)img * nq + x) * L + row] = v[m];                   // [img][q][t]
+    else R[(((size_t)img * groups + grp) * nq + x) * L + row] = v[m];  // [img][grp][q][t]
   }
 }
 
@@ -987,7 +990,8 @@ static int run_two_pass(const FpArgs& a, const FpCoh& c, float2* B, int generic_
   const size_t lds2 = two_pass_rows_lds<PL>(a.wc);
   rc = allow(k_fp_brows<PL>, lds2);
   if (rc) return rc;
-  dim3 g2((PL::N + G - 1) / G, a.batch);
+  if (generic_layout && c.groups != 1) return fail(OI_ERR_INVALID, "two-pass path: one group in the generic layout");
+  dim3 g2((PL::N + G - 1) / G, a.batch, c.groups);
   hipLaunchKernelGGL(k_fp_brows<PL>, g2, dim3(kBlock), lds2, st, B, c.nkern * c.nchan, a.k0c, a.wc,
                      a.qc, generic_layout, a.tw_l, c.R);
   OI_LAUNCH_CHECK("fp_brows");

@@ -93,7 +93,7 @@ typedef struct oi_geometry {
   int qr, qc;
   int path;       /* 0: generic kernels; 1: compile-time-size kernels, one
                      pass over the coarse grid; 2: compile-time-size kernels,
-                     two-pass coherent sum (coarse 256 or 512)               */
+                     two-pass coherent sum (coarse 144, 256 or 512)          */
 } oi_geometry;
 
 typedef struct oi_plan oi_plan;

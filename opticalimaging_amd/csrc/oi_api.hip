@@ -177,7 +177,8 @@ extern "C" int oi_plan_create(oi_geometry* geo, oi_plan** out) {
     const int need = (2 * band_r - 1) > (2 * band_c - 1) ? 2 * band_r - 1 : 2 * band_c - 1;
     const int L = fp_coarse_choose(need);
     if (L > 0 && L < g.ny && L < g.nx && fp_coherent_rows_fit(L) &&
-        fp_coherent_lds(L, g.wr, g.wc) <= kLdsLimit) {
+        fp_coherent_lds(L, g.wr, g.wc) <= kLdsLimit &&
+        (fp_one_pass_ok(L) || !fp_two_pass_size_ok(L))) {
       g.path = 1;
       g.lr = g.lc = lr = lc = L;
     }
@@ -186,7 +187,11 @@ extern "C" int oi_plan_create(oi_geometry* geo, oi_plan** out) {
   // when the window folds onto them without collisions.
   if (g.path == 0 && !g.direct && geo->lr <= 0 && geo->lc <= 0) {
     const int need = (2 * band_r - 1) > (2 * band_c - 1) ? 2 * band_r - 1 : 2 * band_c - 1;
-    const int L = need > 144 ? fp_two_pass_choose(need) : 0;
+    // coarse grids past one workgroup's reach, and those the one-pass kernel
+    // does not serve when the two-pass table has them (144)
+    const int L1 = fp_coarse_choose(need);
+    const int L = need > 144 ? fp_two_pass_choose(need)
+                             : (L1 > 0 && !fp_one_pass_ok(L1) && fp_two_pass_size_ok(L1) ? L1 : 0);
     if (L > 0 && L < g.ny && L < g.nx && g.wr <= L && g.wc <= L &&
         fp_two_pass_lds(L, g.wc) <= kLdsLimit) {
       g.path = 2;

@@ -48,6 +48,7 @@ bool fp_coarse_size_ok(int l);
 int fp_coarse_choose(int need);          // smallest table size >= need, 0 if none
 size_t fp_coherent_lds(int L, int wr, int wc);
 int fp_coherent_rows_fit(int L);
+bool fp_one_pass_ok(int L);             // the one-pass coherent kernel serves L
 int fp_spectrum(const FpArgs& a, hipStream_t st);  // marks stage 1 between its launches
 int fp_image(const FpArgs& a, const FpCoh& c, hipStream_t st);  // marks stages 3, 4, 5
 int fp_interp(const FpArgs& a, int groups, hipStream_t st);      // marks stages 4, 5

@@ -968,7 +968,7 @@ static int run_band(const FpArgs& a, int groups, hipStream_t st) {
 }
 
 // Large coarse grids of the two-pass coherent sum.
-#define OI_TWO_PASS_PLANS(X) X(256, 16, 16) X(512, 16, 32)
+#define OI_TWO_PASS_PLANS(X) X(144, 12, 12) X(256, 16, 16) X(512, 16, 32)
 
 template <class PL>
 static size_t two_pass_rows_lds(int wc) {
@@ -1039,7 +1039,8 @@ int fp_interp(const FpArgs& a, int groups, hipStream_t st) {
 #define X(NN, A, B, NT) if (a.L == NN) rc = fp::run_band<reg::GFFT<NN, A, B>>(a, groups, st);
   OI_COARSE_PLANS(X)
 #undef X
-#define X(NN, A, B) if (a.L == NN) rc = fp::run_band<reg::GFFT<NN, A, B>>(a, groups, st);
+#define X(NN, A, B) \
+  if (rc == OI_ERR_UNSUPPORTED && a.L == NN) rc = fp::run_band<reg::GFFT<NN, A, B>>(a, groups, st);
   OI_TWO_PASS_PLANS(X)
 #undef X
   if (rc) return rc == OI_ERR_UNSUPPORTED ? fail(rc, "fast path: unsupported coarse size") : rc;
@@ -1087,6 +1088,13 @@ size_t fp_coherent_lds(int L, int wr, int wc) {
   OI_COARSE_PLANS(X)
 #undef X
   return (size_t)-1;
+}
+
+bool fp_one_pass_ok(int L) {
+#define X(NN, A, B, NT) if (L == NN) return NT > 0;
+  OI_COARSE_PLANS(X)
+#undef X
+  return false;
 }
 
 int fp_coherent_rows_fit(int L) {

@@ -1,0 +1,93 @@
+"""The benchmark configuration at its full size, through the product path.
+
+64 seeded 1024^2 binary masks (4096 nm period, 35 % rectangles), annular
+0.5/0.8 source, NA 0.85, 193 nm; TCC on the 101^2 grid in split-bf16; every
+eigenpair kept (K = 120), imaged as paired real-mask kernels.  The float64
+oracle runs on one image; the rest is pinned by properties that hold at any
+size:
+
+* clear field: the image is the constant (n/W) sum_j V_j |P(s_j)|^2, the
+  oracle's closed form (obliquity of CalculateAerialImage.py:64);
+* dark field: the image is zero;
+* circular shift: rolling a mask rolls its image by the same pixels;
+* batch independence: an image computed alone equals the same image inside the
+  batch of 64, bit for bit;
+* the last image of the batch against the float64 Abbe oracle (full-order SOCS
+  equals Abbe; CalculateAerialImage.py:18-103)."""
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+from gpu_cases import rel_err
+
+pytestmark = pytest.mark.gpu
+
+B, N = 64, 1024
+
+
+@pytest.fixture(scope="module")
+def setup(gpu):
+    from opticalimaging_amd import CalculateTCCMatrix, DecomposeTCC_SOCS, engine, workloads
+    masks, period = workloads.socs_1024_masks(batch=B, n=N)
+    src, proj, num = workloads.socs_1024_optics(order=24, grid_halfwidth=50)
+    tcc, fg = CalculateTCCMatrix(src, [period, period], proj, 0.0, num)
+    num.Hopkins_Order = 10 ** 6
+    ks = DecomposeTCC_SOCS(tcc, fg, [N, N], num)._oi_kernels
+    del tcc
+    assert ks.K == 120 and ks.band is not None
+    plan = engine.plan_for(ks, 1, B)
+    dm = torch.as_tensor(masks).to(gpu)
+    img = engine.socs_image(dm, ks, plan=plan)
+    torch.cuda.synchronize()
+    print(f"full size: K = {ks.K} ({ks.for_real_masks().K} paired), plan {plan.info()}", flush=True)
+    return dict(masks=masks, dm=dm, img=img, ks=ks, plan=plan, src=src, proj=proj, num=num,
+                period=period)
+
+
+def test_clear_and_dark_field(setup):
+    from opticalimaging_amd import engine
+    s = setup
+    m = torch.ones((B, N, N), device=s["dm"].device)
+    m[B // 2:] = 0.0
+    img = engine.socs_image(m, s["ks"], plan=s["plan"]).cpu().numpy()
+    want = O.clear_field_intensity(s["src"], s["proj"])
+    clear = img[:B // 2]
+    err = float(np.max(np.abs(clear - want))) / want
+    dark = float(np.max(np.abs(img[B // 2:])))
+    print(f"clear field {clear.min():.7f} .. {clear.max():.7f} (closed form {want:.7f}): {err:.2e}; "
+          f"dark field max {dark:.1e}")
+    assert err < 1e-5
+    assert dark == 0.0
+
+
+def test_shift_equivariance(setup):
+    from opticalimaging_amd import engine
+    s = setup
+    dy, dx = 37, -101
+    rolled = torch.roll(s["dm"], shifts=(dy, dx), dims=(1, 2)).contiguous()
+    img = engine.socs_image(rolled, s["ks"], plan=s["plan"])
+    want = torch.roll(s["img"], shifts=(dy, dx), dims=(1, 2))
+    err = float((img - want).abs().max() / want.abs().max())
+    print(f"shift ({dy}, {dx}) over {B} images: {err:.2e}")
+    assert err < 1e-5
+
+
+def test_batch_independence(setup):
+    from opticalimaging_amd import engine
+    s = setup
+    for i in (0, 17, B - 1):
+        alone = engine.socs_image(s["dm"][i:i + 1].contiguous(), s["ks"], plan=s["plan"])
+        assert torch.equal(alone[0], s["img"][i]), i
+
+
+def test_last_image_matches_float64_abbe(setup):
+    from opticalimaging_amd import workloads
+    s = setup
+    sc = type("S", (), {})()
+    sc.Period_X = sc.Period_Y = s["period"]
+    sc.ScatterField = workloads.field_from_mask(s["masks"][B - 1]).numpy()
+    ref, *_ = O.abbe_image(s["src"], sc, s["proj"], s["num"])
+    err = rel_err(s["img"][B - 1].cpu().numpy(), ref[0])
+    print(f"image {B - 1} of {B} vs float64 Abbe: {err:.2e}")
+    assert err < 1e-5

@@ -1,7 +1,8 @@
 // Register-resident FFTs with compile-time sizes.
 //
 // DFT<N, INV>: an N-point DFT on N values held by one thread, fully unrolled
-// mixed-radix Cooley-Tukey (first factor 4, 2, 3 or 5).  Twiddle factors are
+// mixed-radix (first factor 4, 2, 3 or 5): Good-Thomas prime-factor splits
+// where the two factors are coprime, Cooley-Tukey otherwise.  Twiddle factors are
 // compile-time constants: exp(-+2 pi i m / N) is evaluated by constexpr series
 // in float64 and rounded once, so no table is read and nothing spills.
 //
@@ -128,7 +129,19 @@ struct DFT<5, INV> {
   }
 };
 
-// N = F M, n = M f + m, k = k1 + F k2:
+constexpr int gcd_c(int a, int b) { return b == 0 ? a : gcd_c(b, a % b); }
+// multiplicative inverse of a modulo m (gcd(a, m) = 1), by search
+constexpr int inv_mod(int a, int m) {
+  for (int x = 1; x < m; ++x)
+    if ((a * x) % m == 1) return x;
+  return 0;
+}
+
+// N = F M.  Coprime F, M (6, 10, 12, 15, 20, 30, 36, 60): Good-Thomas
+// prime-factor form, no twiddle factors (Chinese-remainder index maps):
+//   n = (M f + F m) mod N,  k = (k1 M (M^-1 mod F) + k2 F (F^-1 mod M)) mod N,
+//   X[k] = sum_m w_M^{m k2} sum_f w_F^{f k1} x[n].
+// Otherwise Cooley-Tukey, n = M f + m, k = k1 + F k2:
 //   Y[m][k1] = sum_f x[M f + m] w_F^{f k1};  Y[m][k1] *= w_N^{m k1};
 //   X[k1 + F k2] = sum_m Y[m][k1] w_M^{m k2}.
 template <int N, bool INV>
@@ -138,6 +151,28 @@ struct DFT {
     constexpr int M = N / F;
     static_assert(F > 1 && F <= 5 && M * F == N, "DFT size must be a product of 2, 3 and 5");
     float2 y[N];
+    if constexpr (gcd_c(F, M) == 1) {
+      constexpr int A = inv_mod(M % F, F), B = inv_mod(F % M, M);
+#pragma unroll
+      for (int m = 0; m < M; ++m) {
+        float2 t[F];
+#pragma unroll
+        for (int f = 0; f < F; ++f) t[f] = x[(M * f + F * m) % N];
+        DFT<F, INV>::run(t);
+#pragma unroll
+        for (int k1 = 0; k1 < F; ++k1) y[m * F + k1] = t[k1];
+      }
+#pragma unroll
+      for (int k1 = 0; k1 < F; ++k1) {
+        float2 t[M];
+#pragma unroll
+        for (int m = 0; m < M; ++m) t[m] = y[m * F + k1];
+        DFT<M, INV>::run(t);
+#pragma unroll
+        for (int k2 = 0; k2 < M; ++k2) x[(k1 * M * A + k2 * F * B) % N] = t[k2];
+      }
+      return;
+    }
 #pragma unroll
     for (int m = 0; m < M; ++m) {
       float2 t[F];

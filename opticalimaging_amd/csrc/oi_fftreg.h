@@ -181,8 +181,15 @@ struct DFT {
       DFT<F, INV>::run(t);
 #pragma unroll
       for (int k1 = 0; k1 < F; ++k1) {
-        if (m * k1 == 0) {
+        const int e = m * k1;   // twiddle w_N^e, 0 <= e < N
+        if (e == 0) {
           y[m * F + k1] = t[k1];
+        } else if (2 * e == N) {          // -1
+          y[m * F + k1] = make_float2(-t[k1].x, -t[k1].y);
+        } else if (4 * e == N || 4 * e == 3 * N) {
+          // -i (forward, e = N/4) or +i: a swap and a sign, exact, no multiply
+          const bool minus_i = (4 * e == N) != INV;
+          y[m * F + k1] = minus_i ? make_float2(t[k1].y, -t[k1].x) : make_float2(-t[k1].y, t[k1].x);
         } else {
           const float wr = (float)root_re(m * k1, N);
           const float wi = (float)(INV ? -root_im(m * k1, N) : root_im(m * k1, N));

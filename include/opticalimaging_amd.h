@@ -269,6 +269,26 @@ size_t oi_socs_adjoint_work_bytes(const oi_plan* plan, int batch, int nkern);
 int oi_socs_adjoint(oi_plan* plan, const float* mask, const float* grad_image, int batch,
                     const void* kern, int nkern, void* work, float* grad_mask, void* stream);
 
+/* ------------------------------------------------------------------ ILT --
+ * The elementwise passes of one normalised gradient step of the pixel-based
+ * ILT objective sum_b ||I(m_b) - T_b||^2, m = sigmoid(theta), around
+ * oi_socs_image and oi_socs_adjoint (no reference counterpart: the reference
+ * has no mask-optimisation loop).  All arrays are float32 [batch][npix]
+ * device arrays; work holds oi_ilt_work_floats(batch) floats (no
+ * initialisation needed; one call at a time per work array).  Sums run over
+ * a fixed split of each image, so results do not depend on the batch.
+ *
+ * oi_ilt_residual: resid2 = 2 (image - target) (resid2 may be image);
+ *   loss (device float, may be null) = sum over the batch of (image - target)^2.
+ * oi_ilt_update: g = grad_m m (1 - m); per image b,
+ *   theta_b -= lr g_b / (sqrt(sum g_b^2 / npix) + 1e-30);
+ *   m_next (may be m, or null) = sigmoid(theta) after the step.            */
+size_t oi_ilt_work_floats(int batch);
+int oi_ilt_residual(const float* image, const float* target, float* resid2, float* loss,
+                    int batch, size_t npix, float* work, void* stream);
+int oi_ilt_update(float* theta, const float* grad_m, const float* m, float lr, int batch,
+                  size_t npix, float* work, float* m_next, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -16,6 +16,7 @@ from __future__ import annotations
 
 import torch
 
+from . import _native as nat
 from . import engine
 
 __all__ = ["SOCSImage", "aerial_image", "kernel_set", "ilt_loss", "ilt"]
@@ -75,30 +76,36 @@ def ilt(theta: torch.Tensor, target: torch.Tensor, kernels, steps: int = 50, lr:
     per-step losses tensor on the device).
 
     The gradient is the one ``ilt_loss`` gives through autograd, written out
-    so that each step is a handful of device passes: m = sigmoid(theta),
-    I = SOCS(m), d = I - T, loss = d.d, dL/dm = adjoint(m, 2 d),
-    dL/dtheta = dL/dm m (1 - m).  The loop stays on the device (no host
-    synchronisation per step)."""
+    so that each step is five device passes: I = SOCS(m); d = 2 (I - T) with
+    the loss (oi_ilt_residual); dL/dm = adjoint(m, d); then dL/dtheta =
+    dL/dm m (1 - m), its per-image RMS, the update of theta and the next
+    m = sigmoid(theta) (oi_ilt_update).  The loop stays on the device (no
+    host synchronisation per step)."""
     ks = kernel_set(kernels)
     theta = theta.detach().to(torch.float32).contiguous().clone()
     if theta.dim() != 3:
         raise ValueError("theta must be (B, Ny, Nx)")
     target = target.detach().to(device=theta.device, dtype=torch.float32).contiguous()
+    if target.shape != theta.shape:
+        raise ValueError("target must have theta's shape")
     B = int(theta.shape[0])
-    n = float(theta[0].numel())
-    plan = engine.plan_for(ks, 1, B)
-    losses = torch.empty(steps, dtype=torch.float32, device=theta.device)
+    npix = int(theta[0].numel()) if B else 0
+    plan = engine.plan_for(ks, 1, max(B, 1))
+    lib = nat.load()
+    losses = torch.zeros(steps, dtype=torch.float32, device=theta.device)
+    if B == 0:
+        return theta, losses
+    work = torch.empty(lib.oi_ilt_work_floats(B), dtype=torch.float32, device=theta.device)
     img = torch.empty_like(theta)
     d = torch.empty_like(theta)
+    g = torch.empty_like(theta)
+    m = torch.sigmoid(theta)
+    st = nat.stream()
     for i in range(steps):
-        m = torch.sigmoid(theta)
         engine.socs_image(m, ks, out=img, plan=plan)
-        torch.sub(img, target, out=d)
-        flat = d.view(-1)
-        losses[i] = torch.dot(flat, flat)
-        d.mul_(2.0)
-        g = engine.socs_adjoint(m, d, ks, plan=plan)
-        g.mul_(m).mul_(1.0 - m)
-        rms = torch.linalg.vector_norm(g.view(B, -1), dim=1) / n ** 0.5 + 1e-30
-        theta.addcdiv_(g, rms.view(B, 1, 1), value=-lr)
+        nat.check(lib.oi_ilt_residual(nat.ptr(img), nat.ptr(target), nat.ptr(d),
+                                      nat.ptr(losses[i:i + 1]), B, npix, nat.ptr(work), st))
+        engine.socs_adjoint(m, d, ks, out=g, plan=plan)
+        nat.check(lib.oi_ilt_update(nat.ptr(theta), nat.ptr(g), nat.ptr(m), float(lr), B, npix,
+                                    nat.ptr(work), nat.ptr(m), st))
     return theta, losses

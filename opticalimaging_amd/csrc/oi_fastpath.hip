@@ -90,10 +90,11 @@ __global__ __launch_bounds__(kBlock) void k_fp_rows_real(const float* __restrict
   }
 }
 
-// Row kernel for a window inside [-T, T] (T = P::T; every image plan has
-// U2 == 1).  After stage 1 and the exchange, thread t only needs outputs 0, 1
-// and V - 1 of its radix-V stage-2 DFT: X[t], X[t + T] and X[N - T + t], i.e.
-// kappa = t, T and t - T.  They are formed directly (one sum and two
+// Row kernel for a window inside [-H T, H T] (T = P::T, H = 1 or 2; every
+// image plan has U2 == 1).  After stage 1 and the exchange, thread t only
+// needs outputs 0 .. H and V - H .. V - 1 of its radix-V stage-2 DFT, i.e.
+// kappa = t + T m (m < H), H T and t - T (m + 1).  For H = 1 these are X[t],
+// X[t + T] and X[N - T + t].  They are formed directly (one sum and H pairs of
 // twiddled sums) instead of the full transform, and the Hermitian partner
 // Z[-kappa] of the two packed rows comes from lane (T - t) mod T by a
 // cross-lane read, so the outputs never go back through LDS.  A block walks
@@ -121,16 +122,35 @@ __device__ __forceinline__ void rows_separate(float2 zk, float2 zm, float2* dst)
   dst[1] = make_float2(0.5f * (zk.y + zm.y), -0.5f * (zk.x - zm.x));    // (Z[k] - conj Z[-k]) / 2i
 }
 
-// Inverse transform (e^{+2 pi i k x / N}) of v, keeping X[t], X[t + T] and
-// X[N - T + t] of the group's thread t: stage 1 as GFFT::run_gen, the
-// exchange, then outputs 0, 1 and V - 1 of the stage-2 DFT only,
-// sum_j b_j e^{+2 pi i j k / V}.  Every thread of the block calls (barrier).
-template <class P>
+// Inverse transform (e^{+2 pi i k x / N}) of v, keeping the stage-2 outputs
+// m = 0 .. H and V - H .. V - 1 of the group's thread t (X[t + T m]): stage 1
+// as GFFT::run_gen, the exchange, then those outputs of the stage-2 DFT only,
+// sum_j b_j e^{+2 pi i j m / V}, the pair (m, V - m) from one set of sums.
+//   pos[m] = X[t + T m]        (m = 0 .. H)
+//   neg[i] = X[t + T (V-1-i)]  (i = 0 .. H-1), i.e. kappa = t - T (i + 1)
+// Every thread of the block calls (barrier).
+// cos and sin of 2 pi j (p + 1) / V for H > 1 output pairs, a constexpr
+// object so that no series is left for the device to evaluate (for H = 1 the
+// inline root_re / root_im fold to immediates on their own)
+template <int V, int H>
+struct NarrowRoots {
+  float c[V][H], s[V][H];
+  constexpr NarrowRoots() : c(), s() {
+    for (int j = 0; j < V; ++j)
+      for (int p = 0; p < H; ++p) {
+        c[j][p] = (float)reg::root_re((long)j * (p + 1), V);
+        s[j][p] = (float)(-reg::root_im((long)j * (p + 1), V));
+      }
+  }
+};
+
+template <class P, int H>
 __device__ __forceinline__ void narrow_inverse(const float2 (&v)[P::V], float2* scr, int t,
-                                               const float2* __restrict__ twt, float2& o0,
-                                               float2& o1, float2& om) {
+                                               const float2* __restrict__ twt,
+                                               float2 (&pos)[H + 1], float2 (&neg)[H]) {
   constexpr int TT = P::T, V = P::V, R1 = P::R1, U1 = P::U1;
   static_assert(P::U2 == 1 && P::R2 == V && TT == R1, "radix-V second stage");
+  static_assert(H >= 1 && 2 * H < V, "output pairs inside one transform");
 #pragma unroll
   for (int u = 0; u < U1; ++u) {
     float2 b[R1];
@@ -142,7 +162,9 @@ __device__ __forceinline__ void narrow_inverse(const float2 (&v)[P::V], float2* 
     for (int j = 0; j < R1; ++j) scr[P::xi(i * R1 + j)] = b[j];
   }
   __syncthreads();
-  float sx = 0.f, sy = 0.f, A = 0.f, B = 0.f, C = 0.f, D = 0.f;
+  float sx = 0.f, sy = 0.f, A[H], B[H], C[H], D[H];
+#pragma unroll
+  for (int p = 0; p < H; ++p) A[p] = B[p] = C[p] = D[p] = 0.f;
 #pragma unroll
   for (int j = 0; j < V; ++j) {
     float2 b = scr[P::xi(t + j * R1)];
@@ -150,20 +172,37 @@ __device__ __forceinline__ void narrow_inverse(const float2 (&v)[P::V], float2* 
       const float2 w = twt[j * t];                  // exp(-2 pi i j t / N), used conjugated
       b = reg::mul(b, make_float2(w.x, -w.y));
     }
-    const float c = (float)reg::root_re(j, V), s = (float)(-reg::root_im(j, V));
     sx += b.x;
     sy += b.y;
-    A = fmaf(b.x, c, A);
-    B = fmaf(b.y, s, B);
-    C = fmaf(b.x, s, C);
-    D = fmaf(b.y, c, D);
+#pragma unroll
+    for (int p = 0; p < H; ++p) {
+      float c, s;
+      if constexpr (H == 1) {
+        c = (float)reg::root_re(j, V);
+        s = (float)(-reg::root_im(j, V));
+      } else {
+        constexpr NarrowRoots<V, H> kRoots{};
+        c = kRoots.c[j][p];
+        s = kRoots.s[j][p];
+      }
+      A[p] = fmaf(b.x, c, A[p]);
+      B[p] = fmaf(b.y, s, B[p]);
+      C[p] = fmaf(b.x, s, C[p]);
+      D[p] = fmaf(b.y, c, D[p]);
+    }
   }
-  o0 = make_float2(sx, sy);
-  o1 = make_float2(A - B, C + D);
-  om = make_float2(A + B, D - C);
+  pos[0] = make_float2(sx, sy);
+#pragma unroll
+  for (int p = 0; p < H; ++p) {
+    pos[p + 1] = make_float2(A[p] - B[p], C[p] + D[p]);   // m = p + 1
+    neg[p] = make_float2(A[p] + B[p], D[p] - C[p]);       // m = V - 1 - p
+  }
 }
 
-template <class P>
+// Orders kappa in [-H T, H T] of two real rows per transform.  The Hermitian
+// partner Z[-kappa] of kappa = t + T m (t > 0) is output V - 1 - m of lane
+// T - t, read across lanes; for t = 0 it is output V - m of the same lane.
+template <class P, int H>
 __global__ __launch_bounds__(kBlock) void k_fp_rows_real_narrow(const float* __restrict__ mask,
                                                                 int ny, int k0c, int wc,
                                                                 const float2* __restrict__ twt,
@@ -194,14 +233,22 @@ __global__ __launch_bounds__(kBlock) void k_fp_rows_real_narrow(const float* __r
 #pragma unroll
     for (int m = 0; m < V; ++m) v[m] = nxt[m];
     if (it + 1 < kRowSets && set + 1 < sets) rows_load_pair<P>(nxt, src, 2 * ((set + 1) * G + g), t);
-    float2 o0, o1, om;                              // X[t], X[t + T], X[N - T + t]
-    narrow_inverse<P>(v, scr, t, twt, o0, o1, om);
-    const float2 p0 = make_float2(__shfl(o0.x, partner), __shfl(o0.y, partner));
-    const float2 pm = make_float2(__shfl(om.x, partner), __shfl(om.y, partner));
-    const int ja = t - k0c, jb = t - TT - k0c, jc = TT - k0c;
-    if (ja >= 0 && ja < wc) rows_separate(o0, t == 0 ? o0 : pm, tile + ja * TS + 2 * g);   // kappa = t
-    if (jb >= 0 && jb < wc) rows_separate(om, t == 0 ? o1 : p0, tile + jb * TS + 2 * g);   // t - T
-    if (t == 0 && jc >= 0 && jc < wc) rows_separate(o1, om, tile + jc * TS + 2 * g);       // T
+    float2 pos[H + 1], neg[H];
+    narrow_inverse<P, H>(v, scr, t, twt, pos, neg);
+#pragma unroll
+    for (int m = 0; m < H; ++m) {
+      // kappa = t + T m; partner: lane T - t's neg[m], or (t = 0) pos[0] / neg[m - 1]
+      const float2 pn = make_float2(__shfl(neg[m].x, partner), __shfl(neg[m].y, partner));
+      const int j = t + TT * m - k0c;
+      if (j >= 0 && j < wc)
+        rows_separate(pos[m], t == 0 ? (m == 0 ? pos[0] : neg[m - 1]) : pn, tile + j * TS + 2 * g);
+      // kappa = t - T (m + 1); partner: lane T - t's pos[m], or (t = 0) pos[m + 1]
+      const float2 pp = make_float2(__shfl(pos[m].x, partner), __shfl(pos[m].y, partner));
+      const int jn = t - TT * (m + 1) - k0c;
+      if (jn >= 0 && jn < wc) rows_separate(neg[m], t == 0 ? pos[m + 1] : pp, tile + jn * TS + 2 * g);
+    }
+    const int jh = TT * H - k0c;                   // kappa = H T (t = 0); partner neg[H - 1]
+    if (t == 0 && jh >= 0 && jh < wc) rows_separate(pos[H], neg[H - 1], tile + jh * TS + 2 * g);
     __syncthreads();
     const int rb = 2 * set * G;
     for (int e = threadIdx.x; e < wc * 2 * G; e += kBlock) {
@@ -282,9 +329,10 @@ __global__ __launch_bounds__(kBlock) void k_fp_cols(const float2* __restrict__ T
   }
 }
 
-// Column kernel for a window inside [-T, T] (the spectrum rows k0r ..
-// k0r + wr - 1): only outputs kappa = t, t - T and T are formed.
-template <class P>
+// Column kernel for a window inside [-H T, H T] (the spectrum rows k0r ..
+// k0r + wr - 1): only outputs kappa = t + T m (m < H), t - T (m + 1) and H T
+// are formed.
+template <class P, int H>
 __global__ __launch_bounds__(kBlock) void k_fp_cols_narrow(const float2* __restrict__ T, int k0r,
                                                            int wr, int wc, float scale,
                                                            const float2* __restrict__ twt,
@@ -300,14 +348,18 @@ __global__ __launch_bounds__(kBlock) void k_fp_cols_narrow(const float2* __restr
   float2 v[V];
 #pragma unroll
   for (int m = 0; m < V; ++m) v[m] = col ? src[t + TT * m] : make_float2(0.f, 0.f);
-  float2 o0, o1, om;
-  narrow_inverse<P>(v, lds + g * P::LDS, t, twt, o0, o1, om);
+  float2 pos[H + 1], neg[H];
+  narrow_inverse<P, H>(v, lds + g * P::LDS, t, twt, pos, neg);
   if (!col) return;
   float2* dst = spec + (size_t)bc * wr * wc + j;
-  const int ia = t - k0r, ib = t - TT - k0r, ic = TT - k0r;
-  if (ia >= 0 && ia < wr) dst[(size_t)ia * wc] = make_float2(o0.x * scale, o0.y * scale);
-  if (ib >= 0 && ib < wr) dst[(size_t)ib * wc] = make_float2(om.x * scale, om.y * scale);
-  if (t == 0 && ic >= 0 && ic < wr) dst[(size_t)ic * wc] = make_float2(o1.x * scale, o1.y * scale);
+#pragma unroll
+  for (int m = 0; m < H; ++m) {
+    const int ia = t + TT * m - k0r, ib = t - TT * (m + 1) - k0r;
+    if (ia >= 0 && ia < wr) dst[(size_t)ia * wc] = make_float2(pos[m].x * scale, pos[m].y * scale);
+    if (ib >= 0 && ib < wr) dst[(size_t)ib * wc] = make_float2(neg[m].x * scale, neg[m].y * scale);
+  }
+  const int ic = TT * H - k0r;
+  if (t == 0 && ic >= 0 && ic < wr) dst[(size_t)ic * wc] = make_float2(pos[H].x * scale, pos[H].y * scale);
 }
 
 // -------------------------------------------------------------- coherent
@@ -830,18 +882,27 @@ static int run_rows(const FpArgs& a, hipStream_t st) {
   constexpr int G = groups_per_block<P>();
   // transform scratch + output tile [wc][2G + 1]
   const size_t lds = rows_lds<P>() + (size_t)a.wc * (2 * G + 1) * sizeof(float2);
-  bool narrow = false;
-  if constexpr (P::STAGES == 2)
-    narrow = a.kind == OI_FIELD_REAL && a.k0c >= -P::T && a.k0c + a.wc - 1 <= P::T &&
-             a.ny % (2 * G) == 0;
+  // narrow form: H output pairs per thread cover the window [-H T, H T]
+  int narrow = 0;
+  if constexpr (P::STAGES == 2) {
+    if (a.kind == OI_FIELD_REAL && a.ny % (2 * G) == 0) {
+      for (int h = 1; h <= 2 && !narrow; ++h)
+        if (2 * h < P::V && a.k0c >= -h * P::T && a.k0c + a.wc - 1 <= h * P::T) narrow = h;
+    }
+  }
   if (narrow) {
     if constexpr (P::STAGES == 2) {
-      int rc = allow(k_fp_rows_real_narrow<P>, lds);
-      if (rc) return rc;
       const int sets = (a.ny + 2 * G - 1) / (2 * G);
       dim3 grid((sets + kRowSets - 1) / kRowSets, a.batch);
-      hipLaunchKernelGGL(k_fp_rows_real_narrow<P>, grid, dim3(kBlock), lds, st,
-                         static_cast<const float*>(a.field), a.ny, a.k0c, a.wc, a.tw_x, a.T);
+      auto go = [&](auto kernel) -> int {
+        int rc = allow(kernel, lds);
+        if (rc) return rc;
+        hipLaunchKernelGGL(kernel, grid, dim3(kBlock), lds, st, static_cast<const float*>(a.field),
+                           a.ny, a.k0c, a.wc, a.tw_x, a.T);
+        return OI_OK;
+      };
+      int rc = narrow == 1 ? go(k_fp_rows_real_narrow<P, 1>) : go(k_fp_rows_real_narrow<P, 2>);
+      if (rc) return rc;
     }
   } else if (a.kind == OI_FIELD_REAL) {
     int rc = allow(k_fp_rows_real<P>, lds);
@@ -876,7 +937,9 @@ static int run_cols(const FpArgs& a, hipStream_t st) {
     return OI_OK;
   };
   if constexpr (P::STAGES == 2) {
-    if (a.k0r >= -P::T && a.k0r + a.wr - 1 <= P::T) return go(k_fp_cols_narrow<P>);
+    if (a.k0r >= -P::T && a.k0r + a.wr - 1 <= P::T) return go(k_fp_cols_narrow<P, 1>);
+    if (4 < P::V && a.k0r >= -2 * P::T && a.k0r + a.wr - 1 <= 2 * P::T)
+      return go(k_fp_cols_narrow<P, 2>);
   }
   return go(k_fp_cols<P>);
 }

@@ -29,6 +29,9 @@
 //   k_adj_ksum    Q = sum_k Ak
 //   k_adj_gcols / k_adj_grows  window -> image grid, real part, two rows per
 //                 complex transform (Hermitian packing)
+// On image sizes with register transforms the last three are replaced by
+// k_adj_fold (sum over k and the Hermitian part, in the band layout of the
+// forward interpolation) and the forward's k_fp_ucols / k_fp_urows.
 #include "oi_adjoint.h"
 #include "oi_fastpath.h"
 #include "oi_fftreg.h"
@@ -360,6 +363,33 @@ __global__ void k_adj_ksum(const float2* __restrict__ Ak, int nk, int per, int b
   }
 }
 
+// The gradient's spectrum folded for the forward interpolation kernels
+// (k_fp_ucols / k_fp_urows, which sum F[q][qr] e^{-2 pi i (qr y / ny + q x / nx)}
+// over the Hermitian extension F[-q][-qr] = conj F[q][qr]):
+//   grad = s Re sum Qw(kr, kc) e^{+2 pi i (kr y / ny + kc x / nx)}
+//   F[b][q][qr + qrm] = s/2 (conj Qw(qr, q) + Qw(-qr, -q)),  q in [0, qcm], |qr| <= qrm,
+// with Qw(kr, kc) = sum_k Ak[b][k] at window orders (kr, kc), zero outside.
+__global__ void k_adj_fold(const float2* __restrict__ Ak, int nk, int k0r, int wr, int k0c, int wc,
+                           int qrm, int qcm, float s, int batch, float2* __restrict__ F) {
+  const int nb = 2 * qrm + 1;
+  const size_t per = (size_t)(qcm + 1) * nb, n = (size_t)batch * per, win = (size_t)wr * wc;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n;
+       e += (size_t)gridDim.x * blockDim.x) {
+    const size_t b = e / per, rem = e - b * per;
+    const int q = (int)(rem / nb), qr = (int)(rem - (size_t)q * nb) - qrm;
+    const float2* ab = Ak + b * nk * win;
+    float2 a = make_float2(0.f, 0.f), c = make_float2(0.f, 0.f);
+    int r = qr - k0r, cc = q - k0c;
+    if (r >= 0 && r < wr && cc >= 0 && cc < wc)
+      for (int k = 0; k < nk; ++k) a = cadd(a, ab[k * win + (size_t)r * wc + cc]);
+    r = -qr - k0r;
+    cc = -q - k0c;
+    if (r >= 0 && r < wr && cc >= 0 && cc < wc)
+      for (int k = 0; k < nk; ++k) c = cadd(c, ab[k * win + (size_t)r * wc + cc]);
+    F[e] = make_float2(0.5f * s * (a.x + c.x), 0.5f * s * (c.y - a.y));
+  }
+}
+
 // Z[b][kc][y] = sum_kr Q[b][kr][kc] exp(+2 pi i (k0r + kr) y / ny)   (wr <= ny)
 __global__ __launch_bounds__(kThreads) void k_adj_gcols(const float2* __restrict__ Q, int k0r,
                                                         int wr, int wc, int ny, int ncol,
@@ -428,6 +458,12 @@ __global__ __launch_bounds__(kThreads) void k_adj_grows(const float2* __restrict
 }
 
 // ------------------------------------------------------------- launchers
+// largest |order| of a window [k0, k0 + w - 1]
+static int adj_half(int k0, int w) {
+  const int a = -k0, b = k0 + w - 1;
+  return a > b ? a : b;
+}
+
 size_t adjoint_work_bytes(const AdjGeom& g, int batch, int nkern) {
   const size_t B = (size_t)batch, K = (size_t)nkern;
   const size_t gq = g.direct ? 0 : (size_t)(2 * g.qr + 1) * (2 * g.qc + 1);
@@ -440,7 +476,10 @@ size_t adjoint_work_bytes(const AdjGeom& g, int batch, int nkern) {
   n += B * K * g.wr * g.wc * sizeof(float2);                         // Ak
   n += B * g.wr * g.wc * sizeof(float2);                             // Q
   n += B * g.wc * g.ny * sizeof(float2);                             // Z
-  return n + 8 * 256;
+  const size_t qrm = adj_half(g.k0r, g.wr), qcm = adj_half(g.k0c, g.wc);
+  n += B * (qcm + 1) * (2 * qrm + 1) * sizeof(float2);               // F
+  n += B * (qcm + 1) * g.ny * sizeof(float2);                        // U
+  return n + 10 * 256;
 }
 
 static char* carve(char*& p, size_t bytes) {
@@ -462,6 +501,9 @@ int launch_adjoint(const AdjGeom& g, const float* gI, const float2* spec, int ba
   float2* Ak = (float2*)carve(w, B * K * g.wr * g.wc * sizeof(float2));
   float2* Q = (float2*)carve(w, B * g.wr * g.wc * sizeof(float2));
   float2* Z = (float2*)carve(w, B * g.wc * g.ny * sizeof(float2));
+  const int qrm = adj_half(g.k0r, g.wr), qcm = adj_half(g.k0c, g.wc);
+  float2* F = (float2*)carve(w, B * (qcm + 1) * (2 * qrm + 1) * sizeof(float2));
+  float2* U = (float2*)carve(w, B * (qcm + 1) * g.ny * sizeof(float2));
 
   // register transforms on the coarse grid when both sizes are tabled and
   // the window (and gamma's band) fold onto it without collisions
@@ -550,6 +592,29 @@ int launch_adjoint(const AdjGeom& g, const float* gI, const float2* spec, int ba
                        g.wc, g.lr, ncol_r, g.flr, Ak);
     OI_LAUNCH_CHECK("adj_cwin");
   }
+  const float scale = 2.0f / ((float)g.lr * (float)g.lc);
+  if (fp_image_size_ok(g.ny) && fp_image_size_ok(g.nx) && 2 * qrm + 1 <= g.ny &&
+      2 * qcm + 1 <= g.nx) {
+    // register transforms of the image sizes: the kernel sum folded into the
+    // band layout of the forward interpolation, then its two kernels
+    const size_t nf = B * (qcm + 1) * (2 * qrm + 1);
+    const size_t nblk = (nf + kThreads - 1) / kThreads;
+    hipLaunchKernelGGL(k_adj_fold, dim3((unsigned)(nblk < 4096 ? nblk : 4096)), dim3(kThreads), 0,
+                       st, Ak, nkern, g.k0r, g.wr, g.k0c, g.wc, qrm, qcm, scale, batch, F);
+    OI_LAUNCH_CHECK("adj_fold");
+    FpArgs a{};
+    a.batch = batch;
+    a.ny = g.ny;
+    a.nx = g.nx;
+    a.qr = qrm;
+    a.qc = qcm;
+    a.tw_x = g.fx.tw;
+    a.tw_y = g.fy.tw;
+    a.Ihat = F;
+    a.U = U;
+    a.out = grad;
+    return fp_band_image(a, st);
+  }
   const int per = g.wr * g.wc;
   const int nblk = (int)((B * per + kThreads - 1) / kThreads);
   hipLaunchKernelGGL(k_adj_ksum, dim3(nblk < 4096 ? nblk : 4096), dim3(kThreads), 0, st, Ak, nkern,
@@ -569,7 +634,6 @@ int launch_adjoint(const AdjGeom& g, const float* gI, const float2* spec, int ba
   if ((rc = allow_lds_adj(k_adj_grows, lds_x))) return rc;
   const int pairs = (g.ny + 1) / 2;
   dim3 g6((pairs + npair - 1) / npair, batch);
-  const float scale = 2.0f / ((float)g.lr * (float)g.lc);
   hipLaunchKernelGGL(k_adj_grows, g6, dim3(kThreads), lds_x, st, Z, g.k0c, g.wc, g.ny, g.nx, npair,
                      g.fx, scale, grad);
   OI_LAUNCH_CHECK("adj_grows");

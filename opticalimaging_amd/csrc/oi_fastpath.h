@@ -52,6 +52,8 @@ bool fp_one_pass_ok(int L);             // the one-pass coherent kernel serves L
 int fp_spectrum(const FpArgs& a, hipStream_t st);  // marks stage 1 between its launches
 int fp_image(const FpArgs& a, const FpCoh& c, hipStream_t st);  // marks stages 3, 4, 5
 int fp_interp(const FpArgs& a, int groups, hipStream_t st);      // marks stages 4, 5
+// interpolation column and row transforms of a band already in a.Ihat
+int fp_band_image(const FpArgs& a, hipStream_t st);
 // Two-pass coherent sum for coarse grids L in {256, 512}: B holds
 // batch * nkern * nchan * wc * L c64; R in [img][q][t] (generic_layout = 0) or
 // [img][t][q] (1).

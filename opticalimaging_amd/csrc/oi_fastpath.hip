@@ -1121,6 +1121,22 @@ int fp_interp(const FpArgs& a, int groups, hipStream_t st) {
   if (a.mark) a.mark(a.mark_ctx, 5);
   return OI_OK;
 }
+// The image from a band Ihat[img][q][2qr+1] (q in [0, qc]), without the
+// band kernel: the interpolation column and row transforms only.
+int fp_band_image(const FpArgs& a, hipStream_t st) {
+  int rc = OI_ERR_UNSUPPORTED;
+#define X(NN, PT) if (a.ny == NN) rc = fp::run_ucols<fp::PT>(a, st);
+  OI_IMAGE_PLANS(X)
+#undef X
+  if (rc) return rc == OI_ERR_UNSUPPORTED ? fail(rc, "fast path: unsupported ny") : rc;
+  rc = OI_ERR_UNSUPPORTED;
+#define X(NN, PT) if (a.nx == NN) rc = fp::run_urows<fp::PT>(a, st);
+  OI_IMAGE_PLANS(X)
+#undef X
+  if (rc) return rc == OI_ERR_UNSUPPORTED ? fail(rc, "fast path: unsupported nx") : rc;
+  return OI_OK;
+}
+
 bool fp_image_size_ok(int n) {
 #define X(NN, PT) if (n == NN) return true;
   OI_IMAGE_PLANS(X)

@@ -38,6 +38,7 @@ def _oracle_grads(masks, gI, kern, pxx, pyy, NA, lam):
 
 CASES = [
     (128, 128, 1000.0, 1000.0, 24),   # compile-time-size path
+    (128, 256, 2000.0, 1000.0, 20),   # compile-time-size path, non-square
     (96, 160, 800.0, 1200.0, 16),     # generic kernels, non-square
     (32, 32, 1000.0, 1000.0, 30),     # coarse grid = image grid (direct)
     (60, 90, 900.0, 1300.0, 10),      # generic, 2-3-5 sizes

@@ -46,6 +46,7 @@ __global__ __launch_bounds__(kIltThreads) void k_ilt_residual(const float* img, 
     const f4v* a4 = reinterpret_cast<const f4v*>(img + base);
     const f4v* t4 = reinterpret_cast<const f4v*>(tgt + base);
     f4v* d4 = reinterpret_cast<f4v*>(d + base);
+#pragma unroll 4   // several loads in flight ahead of the running sum
     for (size_t i = blockIdx.x * (size_t)kIltThreads + threadIdx.x; i < n4; i += stride) {
       const f4v r = a4[i] - t4[i];
       acc += r.x * r.x + r.y * r.y + r.z * r.z + r.w * r.w;
@@ -94,6 +95,7 @@ __global__ __launch_bounds__(kIltThreads) void k_ilt_gnorm(const float* __restri
     const size_t n4 = npix / 4;
     const f4v* g4 = reinterpret_cast<const f4v*>(g + base);
     const f4v* m4 = reinterpret_cast<const f4v*>(m + base);
+#pragma unroll 4   // several loads in flight ahead of the running sum
     for (size_t i = blockIdx.x * (size_t)kIltThreads + threadIdx.x; i < n4; i += stride) {
       const f4v gv = g4[i], mv = m4[i];
       const float a = chain(gv.x, mv.x), b = chain(gv.y, mv.y);

@@ -371,22 +371,27 @@ __global__ void k_adj_ksum(const float2* __restrict__ Ak, int nk, int per, int b
 // with Qw(kr, kc) = sum_k Ak[b][k] at window orders (kr, kc), zero outside.
 __global__ void k_adj_fold(const float2* __restrict__ Ak, int nk, int k0r, int wr, int k0c, int wc,
                            int qrm, int qcm, float s, int batch, float2* __restrict__ F) {
-  const int nb = 2 * qrm + 1;
-  const size_t per = (size_t)(qcm + 1) * nb, n = (size_t)batch * per, win = (size_t)wr * wc;
+  const int nb = 2 * qrm + 1, nq = qcm + 1;
+  const size_t per = (size_t)nq * nb, n = (size_t)batch * per, win = (size_t)wr * wc;
+  // q runs fastest over the threads, so both window reads walk along a row
   for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n;
        e += (size_t)gridDim.x * blockDim.x) {
     const size_t b = e / per, rem = e - b * per;
-    const int q = (int)(rem / nb), qr = (int)(rem - (size_t)q * nb) - qrm;
+    const int i = (int)(rem / nq), q = (int)(rem - (size_t)i * nq), qr = i - qrm;
     const float2* ab = Ak + b * nk * win;
     float2 a = make_float2(0.f, 0.f), c = make_float2(0.f, 0.f);
     int r = qr - k0r, cc = q - k0c;
-    if (r >= 0 && r < wr && cc >= 0 && cc < wc)
+    if (r >= 0 && r < wr && cc >= 0 && cc < wc) {
+#pragma unroll 4
       for (int k = 0; k < nk; ++k) a = cadd(a, ab[k * win + (size_t)r * wc + cc]);
+    }
     r = -qr - k0r;
     cc = -q - k0c;
-    if (r >= 0 && r < wr && cc >= 0 && cc < wc)
+    if (r >= 0 && r < wr && cc >= 0 && cc < wc) {
+#pragma unroll 4
       for (int k = 0; k < nk; ++k) c = cadd(c, ab[k * win + (size_t)r * wc + cc]);
-    F[e] = make_float2(0.5f * s * (a.x + c.x), 0.5f * s * (c.y - a.y));
+    }
+    F[(b * nq + q) * nb + i] = make_float2(0.5f * s * (a.x + c.x), 0.5f * s * (c.y - a.y));
   }
 }
 

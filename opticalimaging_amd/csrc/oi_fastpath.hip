@@ -959,6 +959,7 @@ static int run_ucols(const FpArgs& a, hipStream_t st) {
   const int q = (a.qr + 1 + P::T - 1) / P::T;
   if (q <= 2 && 2 * 2 <= P::V) return go(k_fp_ucols<P, 2>);
   if (q <= 3 && 2 * 3 <= P::V) return go(k_fp_ucols<P, 3>);
+  if (q <= 4 && 2 * 4 < P::V) return go(k_fp_ucols<P, 4>);
   return go(k_fp_ucols<P, 0>);
 }
 
@@ -978,6 +979,7 @@ static int run_urows(const FpArgs& a, hipStream_t st) {
   const int q = (a.qc + 1 + P::T - 1) / P::T;
   if (q <= 2 && 2 * 2 <= P::V) return go(k_fp_urows<P, 2>);
   if (q <= 3 && 2 * 3 <= P::V) return go(k_fp_urows<P, 3>);
+  if (q <= 4 && 2 * 4 < P::V) return go(k_fp_urows<P, 4>);
   return go(k_fp_urows<P, 0>);
 }
 

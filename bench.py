@@ -544,7 +544,7 @@ def main():
             log(f"ilt: {json.dumps(line['ilt'])}")
 
         # ---------------- CPU baseline: the float64 oracle, timed on a bounded
-        # sample; its outputs also serve as the checker for the accuracy fields.
+        # sample; its outputs are also the reference of the accuracy fields.
         import oracle as O
         sc = type("S", (), {})()
         sc.Period_X = sc.Period_Y = period

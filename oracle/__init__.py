@@ -2,7 +2,7 @@
 
 TEST INFRASTRUCTURE ONLY.  Nothing in the product package may import, call or
 execute anything under ``oracle/``; only ``tests/``, ``__graft_entry__.smoke()``
-and the ``cpu_baseline`` leg of ``bench.py`` use it, and only as the checker
+and the ``cpu_baseline`` leg of ``bench.py`` use it, and only to check results
 (or as the timed CPU baseline), never as the thing measured or shipped.
 
 It is a float64 restatement of the reference imaging path of

@@ -7,8 +7,8 @@ Checks, for real masks:
     full-rank band plans; batches of two;
   * at 1024^2 (full size), the exact dot-product identity of a quadratic map:
     sum g (I(h + d) - I(h - d)) / 2 = sum grad . d;
-  * the ILT objective ||I(sigmoid(theta)) - T||^2 with the north-star's
-    differentiation settings (1024^2, kernels from the vector quasar optics):
+  * the ILT objective ||I(sigmoid(theta)) - T||^2 at the mask-optimisation
+    workload (16 x 1024^2, 24 kernels from the vector quasar optics):
     gradient against float64 autograd, against central differences
     (eps = 1e-3) at 64 random pixels, and 50 descent steps that lower the loss.
 Tolerance: max |grad - ref| / max |ref| <= 1e-5 (float32 path vs float64).

@@ -67,6 +67,8 @@ def parse():
     ap.add_argument("--serving-rate", type=float, default=3000.0,
                     help="offered Poisson rate of the serving leg (requests/s)")
     ap.add_argument("--ilt-steps", type=int, default=50, help="gradient steps of the ILT leg (0 skips)")
+    ap.add_argument("--ilt-only", action="store_true",
+                    help="only run the ILT leg and print its JSON (profiling runs)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 device copies of MIB MiB (calibrates HBM byte counters)")
     return ap.parse_args()
@@ -308,6 +310,9 @@ def main():
     args = parse()
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
+        return
+    if args.ilt_only:
+        print(json.dumps({"ilt": ilt_leg(max(1, args.ilt_steps))}), flush=True)
         return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:

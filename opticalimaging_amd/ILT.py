@@ -89,12 +89,12 @@ def ilt(theta: torch.Tensor, target: torch.Tensor, kernels, steps: int = 50, lr:
     if target.shape != theta.shape:
         raise ValueError("target must have theta's shape")
     B = int(theta.shape[0])
-    npix = int(theta[0].numel()) if B else 0
-    plan = engine.plan_for(ks, 1, max(B, 1))
-    lib = nat.load()
     losses = torch.zeros(steps, dtype=torch.float32, device=theta.device)
     if B == 0:
         return theta, losses
+    npix = int(theta[0].numel())
+    plan = engine.plan_for(ks, 1, B)
+    lib = nat.load()
     work = torch.empty(lib.oi_ilt_work_floats(B), dtype=torch.float32, device=theta.device)
     img = torch.empty_like(theta)
     d = torch.empty_like(theta)

@@ -284,6 +284,15 @@ int oi_socs_adjoint(oi_plan* plan, const float* mask, const float* grad_image, i
  *   theta_b -= lr g_b / (sqrt(sum g_b^2 / npix) + 1e-30);
  *   m_next (may be m, or null) = sigmoid(theta) after the step.            */
 size_t oi_ilt_work_floats(int batch);
+/* oi_socs_residual: the SOCS image of real masks (as oi_socs_image with
+ * kind OI_FIELD_REAL) fused with oi_ilt_residual: resid2 = 2 (I - target),
+ * loss (device float, may be null) = sum over the batch of (I - target)^2,
+ * computed in the pass that produces I (I itself is not written).  Plans
+ * whose image sizes have no compile-time transforms (oi_geometry.path 0)
+ * return OI_ERR_UNSUPPORTED before anything is launched: use oi_socs_image
+ * and oi_ilt_residual there.  Calls on one plan share its scratch.         */
+int oi_socs_residual(oi_plan* plan, const float* mask, int batch, const void* kern, int nkern,
+                     const float* target, float* resid2, float* loss, void* stream);
 int oi_ilt_residual(const float* image, const float* target, float* resid2, float* loss,
                     int batch, size_t npix, float* work, void* stream);
 int oi_ilt_update(float* theta, const float* grad_m, const float* m, float lr, int batch,

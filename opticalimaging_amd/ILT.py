@@ -76,8 +76,9 @@ def ilt(theta: torch.Tensor, target: torch.Tensor, kernels, steps: int = 50, lr:
     per-step losses tensor on the device).
 
     The gradient is the one ``ilt_loss`` gives through autograd, written out
-    so that each step is five device passes: I = SOCS(m); d = 2 (I - T) with
-    the loss (oi_ilt_residual); dL/dm = adjoint(m, d); then dL/dtheta =
+    so that each step is four device passes: d = 2 (I - T) and the loss with
+    I = SOCS(m) (oi_socs_residual, or oi_socs_image then oi_ilt_residual
+    where the plan has no fused form); dL/dm = adjoint(m, d); then dL/dtheta =
     dL/dm m (1 - m), its per-image RMS, the update of theta and the next
     m = sigmoid(theta) (oi_ilt_update).  The loop stays on the device (no
     host synchronisation per step)."""
@@ -101,10 +102,16 @@ def ilt(theta: torch.Tensor, target: torch.Tensor, kernels, steps: int = 50, lr:
     g = torch.empty_like(theta)
     m = torch.sigmoid(theta)
     st = nat.stream()
+    fused = True
     for i in range(steps):
-        engine.socs_image(m, ks, out=img, plan=plan)
-        nat.check(lib.oi_ilt_residual(nat.ptr(img), nat.ptr(target), nat.ptr(d),
-                                      nat.ptr(losses[i:i + 1]), B, npix, nat.ptr(work), st))
+        # d = 2 (I - T) and the loss, in the pass that produces I where the
+        # plan has one; otherwise the image, then the residual pass
+        if fused:
+            fused = engine.socs_residual(m, ks, target, d, losses[i:i + 1], plan=plan)
+        if not fused:
+            engine.socs_image(m, ks, out=img, plan=plan)
+            nat.check(lib.oi_ilt_residual(nat.ptr(img), nat.ptr(target), nat.ptr(d),
+                                          nat.ptr(losses[i:i + 1]), B, npix, nat.ptr(work), st))
         engine.socs_adjoint(m, d, ks, out=g, plan=plan)
         nat.check(lib.oi_ilt_update(nat.ptr(theta), nat.ptr(g), nat.ptr(m), float(lr), B, npix,
                                     nat.ptr(work), nat.ptr(m), st))

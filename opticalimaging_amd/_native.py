@@ -94,6 +94,8 @@ SIGNATURES = {
     "oi_socs_adjoint": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_int, c_void_p,
                                 c_void_p, c_void_p]),
     "oi_ilt_work_floats": (c_size_t, [c_int]),
+    "oi_socs_residual": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int, c_void_p, c_void_p,
+                                 c_void_p, c_void_p]),
     "oi_ilt_residual": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_size_t, c_void_p,
                                 c_void_p]),
     "oi_ilt_update": (c_int, [c_void_p, c_void_p, c_void_p, c_float, c_int, c_size_t, c_void_p,

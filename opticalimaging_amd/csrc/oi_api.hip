@@ -102,6 +102,7 @@ struct oi_plan {
   float2* U = nullptr;
   float* Ic = nullptr;
   float2* Ihat = nullptr;
+  float* part = nullptr;    // residual sums of the image-row pass, [B][ny]
   float2* Bbuf = nullptr;   // two-pass column fields, grown to the largest kernel count used
   size_t Bcap = 0;
   size_t bytes = 0;
@@ -128,7 +129,7 @@ static void plan_free(oi_plan* p) {
   if (!p) return;
   for (hipEvent_t e : p->ev) (void)hipEventDestroy(e);
   for (void* ptr : {(void*)p->T, (void*)p->spec, (void*)p->R, (void*)p->U, (void*)p->Ic,
-                    (void*)p->Ihat, (void*)p->Bbuf})
+                    (void*)p->Ihat, (void*)p->part, (void*)p->Bbuf})
     if (ptr) (void)hipFree(ptr);
   free_twiddles(p->pool);
   delete p;
@@ -256,6 +257,7 @@ extern "C" int oi_plan_create(oi_geometry* geo, oi_plan** out) {
       {(void**)&p->U, g.direct ? 0 : B * (size_t)g.ny * (g.qc + 1) * sizeof(float2)},
       {(void**)&p->Ic, (g.direct && G > 1) ? B * G * (size_t)lr * lc * sizeof(float) : 0},
       {(void**)&p->Ihat, g.path ? B * (size_t)(g.qc + 1) * (2 * g.qr + 1) * sizeof(float2) : 0},
+      {(void**)&p->part, g.path ? B * (size_t)g.ny * sizeof(float) : 0},
   };
   p->bytes = p->pool.bytes;
   for (auto& w : want) {
@@ -373,12 +375,30 @@ static int spectrum_impl(oi_plan* p, const void* field, int kind, int cin, const
   return rc;
 }
 
+static int coherent_impl(oi_plan* p, const void* spec, int batch, const void* kern, int nkern,
+                         float* out, const float* target, hipStream_t st);
+
 extern "C" int oi_coherent_image(oi_plan* p, const void* spec, int batch, const void* kern,
                                  int nkern, float* out, void* stream) {
+  return coherent_impl(p, spec, batch, kern, nkern, out, nullptr, (hipStream_t)stream);
+}
+
+// The residual form needs the compile-time-size image kernels (their row pass
+// writes 2 (I - target) and its partial sums).
+static bool residual_fused(const oi_plan* p) {
+  const oi_geometry& g = p->g;
+  return (g.path == 1 || g.path == 2) && fp_image_size_ok(g.ny) && fp_image_size_ok(g.nx) &&
+         fp_urows_blocks(g.ny, g.nx) <= g.ny;
+}
+
+// target non-null: out = 2 (I - target), partial sums of (I - target)^2 in p->part
+static int coherent_impl(oi_plan* p, const void* spec, int batch, const void* kern, int nkern,
+                         float* out, const float* target, hipStream_t st) {
   if (!p || !spec || !kern || !out) return fail(OI_ERR_INVALID, "coherent_image: null argument");
+  if (target && !residual_fused(p))
+    return fail(OI_ERR_UNSUPPORTED, "socs_residual: plan without the compile-time-size image pass");
   if (batch < 1 || batch > p->g.max_batch) return fail(OI_ERR_INVALID, "batch outside [1, max_batch]");
   if (nkern < 1) return fail(OI_ERR_INVALID, "coherent_image: nkern must be >= 1");
-  hipStream_t st = (hipStream_t)stream;
   const oi_geometry& g = p->g;
   const int phases = g.lr / g.rows;
   int groups = g.max_groups < nkern ? g.max_groups : nkern;
@@ -398,6 +418,8 @@ extern "C" int oi_coherent_image(oi_plan* p, const void* spec, int batch, const 
     }
     FpArgs a = fast_args(p, batch, st);
     a.out = out;
+    a.target = target;
+    a.part = p->part;
     FpCoh c{};
     c.spec = static_cast<const float2*>(spec);
     c.kern = static_cast<const float2*>(kern);
@@ -443,6 +465,8 @@ extern "C" int oi_coherent_image(oi_plan* p, const void* spec, int batch, const 
   if (g.path) {
     FpArgs a = fast_args(p, batch, st);
     a.out = out;
+    a.target = target;
+    a.part = p->part;
     FpCoh c{};
     c.spec = static_cast<const float2*>(spec);
     c.kern = static_cast<const float2*>(kern);
@@ -554,6 +578,23 @@ extern "C" int oi_socs_image(oi_plan* p, const void* field, int kind, int cin, c
   int rc = oi_spectrum_window(p, field, kind, cin, chans, batch, p ? p->spec : nullptr, stream);
   if (rc) return rc;
   return oi_coherent_image(p, p->spec, batch, kern, nkern, out, stream);
+}
+
+extern "C" int oi_socs_residual(oi_plan* p, const float* mask, int batch, const void* kern,
+                                int nkern, const float* target, float* resid2, float* loss,
+                                void* stream) {
+  if (!p || !mask || !kern || !target || !resid2)
+    return fail(OI_ERR_INVALID, "socs_residual: null argument");
+  if (p->g.nchan != 1) return fail(OI_ERR_INVALID, "socs_residual: real masks need a one-channel plan");
+  if (!residual_fused(p))
+    return fail(OI_ERR_UNSUPPORTED, "socs_residual: plan without the compile-time-size image pass");
+  const int chan0 = 0;
+  int rc = oi_spectrum_window(p, mask, OI_FIELD_REAL, 1, &chan0, batch, p->spec, stream);
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  rc = coherent_impl(p, p->spec, batch, kern, nkern, resid2, target, st);
+  if (rc || !loss) return rc;
+  return ilt_sum(p->part, batch * fp_urows_blocks(p->g.ny, p->g.nx), loss, st);
 }
 
 extern "C" int oi_spectrum_full(const void* field, int kind, int cin, const int* chans, int nchan,

@@ -27,6 +27,8 @@ struct FpArgs {
   float2* Ihat;   // [B][qc+1][2qr+1]
   float2* U;      // [B][qc+1][ny]
   float* out;     // [B][ny][nx]
+  const float* target;  // non-null: out = 2 (I - target), part gets the sums of (I - target)^2
+  float* part;          // [B][fp_urows_blocks(ny, nx)]
   StageMark mark;
   void* mark_ctx;
 };
@@ -54,6 +56,10 @@ int fp_image(const FpArgs& a, const FpCoh& c, hipStream_t st);  // marks stages 
 int fp_interp(const FpArgs& a, int groups, hipStream_t st);      // marks stages 4, 5
 // interpolation column and row transforms of a band already in a.Ihat
 int fp_band_image(const FpArgs& a, hipStream_t st);
+// workgroups per image of the image-row pass (partial sums of its residual form)
+int fp_urows_blocks(int ny, int nx);
+// out[0] = sum of part[0 .. n) in a fixed order (oi_ilt.hip)
+int ilt_sum(const float* part, int n, float* out, hipStream_t st);
 // Two-pass coherent sum for coarse grids L in {256, 512}: B holds
 // batch * nkern * nchan * wc * L c64; R in [img][q][t] (generic_layout = 0) or
 // [img][t][q] (1).

@@ -794,10 +794,15 @@ __global__ __launch_bounds__(kBlock) void k_fp_ucols(const float2* __restrict__ 
 // Q > 0: the band has qc < Q T, so thread t's inputs x = t + T m can be
 // non-zero only for m < Q or m >= V - Q; the other registers are the constant
 // zero and the first-stage arithmetic on them folds away at compile time.
-template <class P, int Q>
+// RES: instead of the image I, write 2 (I - target) and the workgroup's sum
+// of (I - target)^2 to part[img * gridDim.x + blockIdx.x] (the residual of a
+// least-squares fit to target, fused into the pass that produces I).
+template <class P, int Q, bool RES>
 __global__ __launch_bounds__(kBlock) void k_fp_urows(const float2* __restrict__ U, int ny, int qc,
                                                      const float2* __restrict__ twt,
-                                                     float* __restrict__ out) {
+                                                     float* __restrict__ out,
+                                                     const float* __restrict__ target,
+                                                     float* __restrict__ part) {
   constexpr int N = P::N, TT = P::T, V = P::V, G = groups_per_block<P>();
   constexpr int TS = 2 * G + 1;               // tile row stride (odd: lanes reading
                                               // consecutive q hit different banks)
@@ -835,16 +840,56 @@ __global__ __launch_bounds__(kBlock) void k_fp_urows(const float2* __restrict__ 
     }
     v[m] = z;
   }
+  const int ya = y0 + 2 * gg, yb = ya + 1;
+  // residual form: the target rows are loaded before the transform, so their
+  // latency hides behind it (a load issued after the stores would also wait
+  // for them: loads and stores share one in-order counter)
+  float2 tgt[RES ? V : 1];
+  if constexpr (RES) {
+#pragma unroll
+    for (int m = 0; m < V; ++m) {
+      const int x = t + TT * m;
+      tgt[m].x = live && ya < ny ? __builtin_nontemporal_load(target + ((size_t)img * ny + ya) * N + x) : 0.f;
+      tgt[m].y = live && yb < ny ? __builtin_nontemporal_load(target + ((size_t)img * ny + yb) * N + x) : 0.f;
+    }
+  }
   P::template run_live_tab<false>(v, scr + g * P::LDS, t, twt, true);
-  if (!live) return;
+  if (!RES && !live) return;
   // the image is written once and not read by this pass: non-temporal stores
   // stream it to memory instead of leaving dirty lines in the caches
-  const int ya = y0 + 2 * gg, yb = ya + 1;
+  float acc = 0.f;
 #pragma unroll
   for (int m = 0; m < V; ++m) {
     const int x = t + TT * m;
-    if (ya < ny) __builtin_nontemporal_store(v[m].x, out + ((size_t)img * ny + ya) * N + x);
-    if (yb < ny) __builtin_nontemporal_store(v[m].y, out + ((size_t)img * ny + yb) * N + x);
+    const size_t ia = ((size_t)img * ny + ya) * N + x, ib = ((size_t)img * ny + yb) * N + x;
+    if (RES) {
+      if (live && ya < ny) {
+        const float r = v[m].x - tgt[m].x;
+        acc += r * r;
+        __builtin_nontemporal_store(2.f * r, out + ia);
+      }
+      if (live && yb < ny) {
+        const float r = v[m].y - tgt[m].y;
+        acc += r * r;
+        __builtin_nontemporal_store(2.f * r, out + ib);
+      }
+    } else {
+      if (ya < ny) __builtin_nontemporal_store(v[m].x, out + ia);
+      if (yb < ny) __builtin_nontemporal_store(v[m].y, out + ib);
+    }
+  }
+  if constexpr (RES) {
+    __shared__ float red[kBlock / 64];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      float sum = 0.f;
+#pragma unroll
+      for (int w = 0; w < kBlock / 64; ++w) sum += red[w];
+      part[(size_t)img * gridDim.x + blockIdx.x] = sum;
+    }
   }
 }
 
@@ -971,16 +1016,23 @@ static int run_urows(const FpArgs& a, hipStream_t st) {
   auto go = [&](auto kernel) -> int {
     int rc = allow(kernel, lds);
     if (rc) return rc;
-    hipLaunchKernelGGL(kernel, grid, dim3(kBlock), lds, st, a.U, a.ny, a.qc, a.tw_x, a.out);
+    hipLaunchKernelGGL(kernel, grid, dim3(kBlock), lds, st, a.U, a.ny, a.qc, a.tw_x, a.out,
+                       a.target, a.part);
     OI_LAUNCH_CHECK("fp_urows");
     return OI_OK;
   };
   // registers that can hold band values: Q = ceil((qc + 1) / T) at each end
   const int q = (a.qc + 1 + P::T - 1) / P::T;
-  if (q <= 2 && 2 * 2 <= P::V) return go(k_fp_urows<P, 2>);
-  if (q <= 3 && 2 * 3 <= P::V) return go(k_fp_urows<P, 3>);
-  if (q <= 4 && 2 * 4 < P::V) return go(k_fp_urows<P, 4>);
-  return go(k_fp_urows<P, 0>);
+  if (a.target) {
+    if (q <= 2 && 2 * 2 <= P::V) return go(k_fp_urows<P, 2, true>);
+    if (q <= 3 && 2 * 3 <= P::V) return go(k_fp_urows<P, 3, true>);
+    if (q <= 4 && 2 * 4 < P::V) return go(k_fp_urows<P, 4, true>);
+    return go(k_fp_urows<P, 0, true>);
+  }
+  if (q <= 2 && 2 * 2 <= P::V) return go(k_fp_urows<P, 2, false>);
+  if (q <= 3 && 2 * 3 <= P::V) return go(k_fp_urows<P, 3, false>);
+  if (q <= 4 && 2 * 4 < P::V) return go(k_fp_urows<P, 4, false>);
+  return go(k_fp_urows<P, 0, false>);
 }
 
 // The one-pass kernel serves coarse sizes with a block size in the table when
@@ -1137,6 +1189,14 @@ int fp_band_image(const FpArgs& a, hipStream_t st) {
 #undef X
   if (rc) return rc == OI_ERR_UNSUPPORTED ? fail(rc, "fast path: unsupported nx") : rc;
   return OI_OK;
+}
+
+int fp_urows_blocks(int ny, int nx) {
+#define X(NN, PT) \
+  if (nx == NN) return (ny + 2 * fp::groups_per_block<fp::PT>() - 1) / (2 * fp::groups_per_block<fp::PT>());
+  OI_IMAGE_PLANS(X)
+#undef X
+  return 0;
 }
 
 bool fp_image_size_ok(int n) {

@@ -7,6 +7,7 @@
 // batch or on timing.  These passes stream 4-byte values: HBM-bound, float4
 // loads and stores when the pointers and the image size allow.
 #include "oi_common.h"
+#include "oi_fastpath.h"
 #include "opticalimaging_amd.h"
 
 namespace oi {
@@ -161,6 +162,14 @@ __global__ __launch_bounds__(kIltThreads) void k_ilt_update(float* theta, const 
 bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; }
 
 }  // namespace
+}  // namespace oi
+
+namespace oi {
+int ilt_sum(const float* part, int n, float* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_ilt_total, dim3(1), dim3(64), 0, st, part, n, out);
+  OI_LAUNCH_CHECK("ilt_total");
+  return OI_OK;
+}
 }  // namespace oi
 
 using namespace oi;

@@ -296,6 +296,34 @@ def socs_image(masks: torch.Tensor, ks: KernelSet, kind: int = nat.OI_FIELD_REAL
     return out
 
 
+def socs_residual(masks: torch.Tensor, ks: KernelSet, target: torch.Tensor, out: torch.Tensor,
+                  loss: torch.Tensor | None = None, plan: Plan | None = None,
+                  pack_real: bool = True) -> bool:
+    """out = 2 (I - target) for the SOCS image I of real masks [B, ny, nx], and
+    loss[0] = sum (I - target)^2, fused into the pass that produces I
+    (oi_socs_residual).  Returns False, with nothing computed, when the plan
+    has no compile-time-size image pass; socs_image then does the work."""
+    B = int(masks.shape[0])
+    if B == 0:
+        if loss is not None:
+            loss.zero_()
+        return True
+    if (int(masks.shape[1]), int(masks.shape[2])) != (ks.ny, ks.nx):
+        raise ValueError(f"mask grid does not match kernel grid {ks.ny}x{ks.nx}")
+    if target.shape != masks.shape or out.shape != masks.shape:
+        raise ValueError("target and out must have the masks' shape")
+    if pack_real and (ks._real is not None or B >= PAIR_MIN_BATCH):
+        ks = ks.for_real_masks()
+    if plan is None:
+        plan = plan_for(ks, 1, B)
+    rc = nat.load().oi_socs_residual(plan.handle, nat.ptr(masks), B, nat.ptr(ks.win), ks.K,
+                                     nat.ptr(target), nat.ptr(out), nat.ptr(loss), nat.stream())
+    if rc == nat.OI_ERR_UNSUPPORTED:
+        return False
+    nat.check(rc)
+    return True
+
+
 def socs_adjoint(masks: torch.Tensor, grad_image: torch.Tensor, ks: KernelSet,
                  out: torch.Tensor | None = None, plan: Plan | None = None,
                  pack_real: bool = True) -> torch.Tensor:

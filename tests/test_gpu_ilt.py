@@ -117,3 +117,63 @@ def test_ilt_loop_matches_torch_written_loop(gpu):
     terr = float((theta - th).abs().max())
     print(f"ILT loop vs torch-written loop, {steps} steps: losses {lerr:.1e}, theta {terr:.1e}")
     assert lerr < 1e-5 and terr < 1e-4
+
+
+def _residual_case(ks, n, batch, seed):
+    from opticalimaging_amd import engine
+    rng = np.random.default_rng(seed)
+    m = torch.as_tensor(rng.random((batch, n, n)).astype(np.float32)).cuda()
+    T = torch.as_tensor((rng.random((batch, n, n)) < 0.3).astype(np.float32)).cuda()
+    plan = engine.plan_for(ks, 1, batch)
+    d = torch.empty_like(m)
+    loss = torch.full((1,), -1.0, device="cuda")
+    fused = engine.socs_residual(m, ks, T, d, loss, plan=plan)
+    img = engine.socs_image(m, ks, plan=plan)
+    torch.cuda.synchronize()
+    return fused, plan.info(), d, loss, img, T
+
+
+@pytest.mark.parametrize("case", ["one-pass 256", "two-pass 1024"])
+def test_socs_residual_matches_image_then_residual(gpu, case):
+    """oi_socs_residual writes 2 (I - T) from the same image values the plain
+    pass writes (bit for bit) and the loss sum (I - T)^2."""
+    from opticalimaging_amd import engine, workloads
+    from opticalimaging_amd.DecomposeTCC_SOCS import socs_kernel_set
+    if case == "one-pass 256":
+        n, period = 256, 1024.0
+        src, proj, num = annular(), projector(), numerics(order=24)
+        tcc, fg = O.tcc_matrix(src, [period, period], proj, 0.0, num)
+        kern = O.decompose_tcc_socs(tcc, fg, [n, n], num)
+        ks = engine.KernelSet.from_dense(torch.as_tensor(kern.astype(np.complex64)))
+        want_path = 1
+    else:
+        n = 1024
+        src, proj, num = workloads.ilt_optics()
+        ks = socs_kernel_set(src, [4.0 * n, 4.0 * n], proj, 0.0, num, [n, n])
+        want_path = 2
+    fused, info, d, loss, img, T = _residual_case(ks, n, 4, seed=n)
+    assert fused and info["path"] == want_path, info
+    assert torch.equal(d, (img - T) * 2.0)
+    want = float(((img - T).double() ** 2).sum())
+    print(f"{case}: loss {float(loss):.6e} vs {want:.6e}")
+    assert abs(float(loss) - want) / want < 1e-5
+
+
+def test_socs_residual_reports_generic_plans(gpu):
+    """A plan on the generic path (96 x 160) has no fused residual: the call
+    returns False and leaves the outputs untouched."""
+    from opticalimaging_amd import engine
+    ny, nx, pyy, pxx = 96, 160, 800.0, 1200.0
+    src, proj, num = annular(), projector(), numerics(order=16)
+    tcc, fg = O.tcc_matrix(src, [pyy, pxx], proj, 0.0, num)
+    kern = O.decompose_tcc_socs(tcc, fg, [nx, ny], num)
+    ks = engine.KernelSet.from_dense(torch.as_tensor(kern.astype(np.complex64)))
+    m = torch.rand((2, ny, nx), device="cuda")
+    T = torch.rand((2, ny, nx), device="cuda")
+    plan = engine.plan_for(ks, 1, 2)
+    assert plan.info()["path"] == 0
+    d = torch.full_like(m, 7.0)
+    loss = torch.full((1,), -1.0, device="cuda")
+    assert not engine.socs_residual(m, ks, T, d, loss, plan=plan)
+    torch.cuda.synchronize()
+    assert float(loss) == -1.0 and bool((d == 7.0).all())

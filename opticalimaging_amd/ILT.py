@@ -97,7 +97,7 @@ def ilt(theta: torch.Tensor, target: torch.Tensor, kernels, steps: int = 50, lr:
     plan = engine.plan_for(ks, 1, B)
     lib = nat.load()
     work = torch.empty(lib.oi_ilt_work_floats(B), dtype=torch.float32, device=theta.device)
-    img = torch.empty_like(theta)
+    img = None                     # the image itself, only where the residual is not fused
     d = torch.empty_like(theta)
     g = torch.empty_like(theta)
     m = torch.sigmoid(theta)
@@ -109,6 +109,8 @@ def ilt(theta: torch.Tensor, target: torch.Tensor, kernels, steps: int = 50, lr:
         if fused:
             fused = engine.socs_residual(m, ks, target, d, losses[i:i + 1], plan=plan)
         if not fused:
+            if img is None:
+                img = torch.empty_like(theta)
             engine.socs_image(m, ks, out=img, plan=plan)
             nat.check(lib.oi_ilt_residual(nat.ptr(img), nat.ptr(target), nat.ptr(d),
                                           nat.ptr(losses[i:i + 1]), B, npix, nat.ptr(work), st))

@@ -322,7 +322,8 @@ def main():
         sys.exit(2)
 
     from opticalimaging_amd import (CalculateTCCMatrix, DecomposeTCC_SOCS, engine, workloads)
-    from opticalimaging_amd.CalculateTCCMatrix import tcc_contract, precision_code
+    from opticalimaging_amd.CalculateTCCMatrix import factor_of, precision_code, tcc_contract
+    from opticalimaging_amd.DecomposeTCC_SOCS import kernels_of
 
     dev = torch.device("cuda", 0)
     B, N = args.batch, 1024
@@ -340,7 +341,7 @@ def main():
     num.Hopkins_Order = 24               # the configuration's stated truncation
     kern24 = DecomposeTCC_SOCS(tcc, fg, [N, N], num)
     torch.cuda.synchronize()
-    ks_full, ks24 = kern_full._oi_kernels, kern24._oi_kernels
+    ks_full, ks24 = kernels_of(kern_full), kernels_of(kern24)
     log(f"kernel sets: TCC {tuple(tcc.shape)} + two decompositions in {time.perf_counter() - t:.2f}s; "
         f"window {ks_full.wr}x{ks_full.wc} at ({ks_full.k0r},{ks_full.k0c}); K = {ks_full.K} "
         f"(full rank, band {ks_full.band}) and {ks24.K}")
@@ -400,7 +401,8 @@ def main():
         other's streaming stages."""
         br, bc = ks.band if ks.band is not None else (0, 0)
         plans = [engine.Plan(ks.ny, ks.nx, ks.k0r, ks.wr, ks.k0c, ks.wc, nchan=1, max_batch=B,
-                             band_r=min(br, ks.wr), band_c=min(bc, ks.wc)) for _ in range(2)]
+                             band_r=min(br, ks.wr), band_c=min(bc, ks.wc), max_kernels=ks.K)
+                 for _ in range(2)]
         outs = [torch.empty_like(masks) for _ in range(2)]
         streams = [torch.cuda.Stream() for _ in range(2)]
 
@@ -504,7 +506,7 @@ def main():
         log(f"plain GPU Abbe: {json.dumps(pl)}")
 
         # ---------------- TCC contraction on MFMA (M = 10201, Nsrc = 120)
-        f = tcc._oi_factor
+        f = factor_of(tcc)
         M = f.G.shape[-1]
         nsrc = f.G.shape[1]
         res = {}

@@ -68,8 +68,11 @@ int oi_version(void);
  * Geometry of one imaging problem.  Inputs to oi_plan_create: ny, nx, the
  * spectrum window, nchan (channels carried per image), max_batch, max_groups
  * (kernels are split into this many partial sums, fixed for the plan so a
- * result never depends on the batch it is computed in; 0 = choose) and
- * optionally lr, lc (coarse grid; 0 = choose).
+ * result never depends on the batch it is computed in; 0 = choose),
+ * max_kernels (the largest kernel count any call on the plan passes; every
+ * buffer that grows with it is allocated here, and calls with more kernels
+ * fail with OI_ERR_INVALID; 0 = 256) and optionally lr, lc (coarse grid;
+ * 0 = choose).
  * Outputs filled by the library: max_groups (when 0), lr, lc, rows (coarse rows per workgroup),
  * direct (1 when the coarse grid equals the image grid), qr, qc (half-widths
  * of the intensity band carried from the coarse to the image grid), path
@@ -94,6 +97,7 @@ typedef struct oi_geometry {
   int path;       /* 0: generic kernels; 1: compile-time-size kernels, one
                      pass over the coarse grid; 2: compile-time-size kernels,
                      two-pass coherent sum (coarse 144, 256 or 512)          */
+  int max_kernels;
 } oi_geometry;
 
 typedef struct oi_plan oi_plan;
@@ -239,7 +243,10 @@ int oi_kernels_from_gram(const void* G, const double* w, int M, int nsrc,
  *   w : float64 [n] eigenvalues (unsorted, in V's column order)
  *   V : c128 [n][n] eigenvectors as columns
  *   work : oi_eigh_work_bytes(n) bytes; sweeps_out : host int or NULL.
- * Synchronises the stream once per sweep (convergence test).              */
+ * Synchronises the stream once per sweep (convergence test: off-diagonal
+ * Frobenius norm <= tol times the matrix's).  Returns OI_ERR_RUNTIME when a
+ * norm is not finite, or when max_sweeps sweeps leave the off-diagonal norm
+ * above max(tol, 4 n DBL_EPSILON) times the matrix's.                      */
 size_t oi_eigh_work_bytes(int n);
 int oi_eigh_hermitian(void* A, int n, double* w, void* V, int max_sweeps, double tol,
                       void* work, int* sweeps_out, void* stream);

@@ -52,8 +52,9 @@ def abbe_images(field, source, scatter, projector, numerics) -> torch.Tensor:
     src = optics.source_set(source)
     k0r, wr, k0c, wc, br, bc = optics.abbe_window(src, py, px, ny, nx)
     tens, kind, cin, live = engine.field_encoding(h)
+    ncomp = 3 if vector else 1
     plan = engine.PLANS.get(ny=ny, nx=nx, k0r=k0r, wr=wr, k0c=k0c, wc=wc, nchan=len(live),
-                            max_batch=1, band_r=br, band_c=bc)
+                            max_batch=1, band_r=br, band_c=bc, max_kernels=ncomp * src.n)
     lib = nat.load()
     spec = torch.empty((1, len(live), wr, wc), dtype=torch.complex64, device=dev)
     cptr, keep = nat.int_array(live)

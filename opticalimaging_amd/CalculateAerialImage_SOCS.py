@@ -18,27 +18,11 @@ from .Scatter import _pitches, pupil_image
 
 __all__ = ["CalculateAerialImage_SOCS", "CalculateSOCS", "kernel_set_of", "socs_field_image"]
 
-_cache = {}
-
-
 def kernel_set_of(kernels) -> engine.KernelSet:
     """Windowed kernel set for dense SOCS kernels (Ny, Nx, K): the one
-    attached by DecomposeTCC_SOCS, else extracted from the exact support (cached
-    per tensor storage and version)."""
-    if isinstance(kernels, engine.KernelSet):
-        return kernels
-    ks = getattr(kernels, "_oi_kernels", None)
-    if ks is not None:
-        return ks
-    t = torch.as_tensor(kernels)
-    key = (t.data_ptr(), t.device.type, tuple(t.shape), getattr(t, "_version", 0))
-    ks = _cache.get(key)
-    if ks is None:
-        ks = engine.KernelSet.from_dense(t)
-        if len(_cache) > 16:
-            _cache.clear()
-        _cache[key] = ks
-    return ks
+    attached by DecomposeTCC_SOCS while the tensor is unmodified, else
+    extracted from the exact support (engine.kernel_set_of)."""
+    return engine.kernel_set_of(kernels)
 
 
 def socs_field_image(field, ks: engine.KernelSet) -> torch.Tensor:
@@ -100,7 +84,7 @@ def CalculateSOCS(FloquetMode, TCCMatrix_SOCS, numerics):
     live = [c for c in range(3) if bool((spec[c] != 0).any())] or [0]
     spec = (spec[live] * ph[None]).unsqueeze(0).contiguous()          # [1][C][n0][n1]
     plan = engine.PLANS.get(ny=gx, nx=gy, k0r=-ox, wr=n0, k0c=-oy, wc=n1, nchan=len(live),
-                            max_batch=1, band_r=0, band_c=0)
+                            max_batch=1, band_r=0, band_c=0, max_kernels=K)
     out = torch.empty((1, gx, gy), dtype=torch.float32, device=dev)
     nat.check(nat.load().oi_coherent_image(plan.handle, nat.ptr(spec), 1, nat.ptr(kwin), K,
                                            nat.ptr(out), nat.stream()))

@@ -7,9 +7,10 @@ split-bf16 with fp32 accumulation unless ``numerics.TCC_Precision`` says
 'fp32' or 'fp64'.
 
 The returned TCC tensor lives on the device and carries its low-rank factor
-(attribute ``_oi_factor``: G, the source weights and the scale), which
-``DecomposeTCC_SOCS`` uses to take the eigenpairs from the small Gram matrix
-instead of the M x M TCC.
+(G, the source weights and the scale; ``factor_of``), valid while the tensor
+is unmodified.  ``DecomposeTCC_SOCS`` uses it to decompose the TCC on the
+factor's range (float64 Rayleigh-Ritz on an n x n problem) instead of
+solving the M x M eigenproblem.
 """
 from __future__ import annotations
 
@@ -18,10 +19,10 @@ from dataclasses import dataclass
 import torch
 
 from . import _native as nat
-from . import optics
+from . import engine, optics
 
 __all__ = ["CalculateTCCMatrix", "GetTCCMatrix", "TCCFactor", "tcc_factor", "tcc_contract",
-           "cartesian_to_polar"]
+           "cartesian_to_polar", "factor_of"]
 
 _PRECISIONS = {"split_bf16": nat.OI_TCC_SPLIT_BF16, "fp32": nat.OI_TCC_FP32, "fp64": nat.OI_TCC_FP64}
 
@@ -40,6 +41,12 @@ class TCCFactor:
     lenf: int
     leng: int
     pitch: tuple = (0.0, 0.0)   # (p_0, p_1) = xypitch * NA / lambda
+
+
+def factor_of(tcc):
+    """The TCCFactor attached by CalculateTCCMatrix, or None when there is
+    none or the TCC was modified after it was built."""
+    return engine.attached(tcc, "_oi_factor")
 
 
 def precision_code(numerics=None) -> int:
@@ -95,7 +102,8 @@ def CalculateTCCMatrix(source, xypitch, projector, focus, numerics):
     (CalculateTCCMatrix.py:23); f pairs with the source X coordinate."""
     factor = tcc_factor(source, xypitch, projector, focus, numerics)
     tcc = tcc_contract(factor.G, factor.V, factor.alpha, precision_code(numerics))
-    tcc._oi_factor = factor
+    # valid while the TCC is unmodified (engine.attached checks its version)
+    engine.attach(tcc, "_oi_factor", factor)
     return tcc, [factor.leng, factor.lenf]
 
 

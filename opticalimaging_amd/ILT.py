@@ -24,12 +24,7 @@ __all__ = ["SOCSImage", "aerial_image", "kernel_set", "ilt_loss", "ilt"]
 
 def kernel_set(kernels) -> engine.KernelSet:
     """KernelSet from DecomposeTCC_SOCS output (dense (Ny, Nx, K)) or a KernelSet."""
-    if isinstance(kernels, engine.KernelSet):
-        return kernels
-    ks = getattr(kernels, "_oi_kernels", None)
-    if ks is not None:
-        return ks
-    return engine.KernelSet.from_dense(torch.as_tensor(kernels))
+    return engine.kernel_set_of(kernels)
 
 
 class SOCSImage(torch.autograd.Function):

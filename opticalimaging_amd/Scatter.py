@@ -58,7 +58,7 @@ def pupil_image(field: torch.Tensor, px: float, py: float, hk: torch.Tensor | No
     if inside and hk is None:
         tens, kind, cin_, live = engine.field_encoding(h)
         plan = engine.PLANS.get(ny=ny, nx=nx, k0r=-oy, wr=2 * oy, k0c=-ox, wc=2 * ox,
-                                nchan=len(live), max_batch=1)
+                                nchan=len(live), max_batch=1, max_kernels=1)
         spec = torch.empty((1, len(live), 2 * oy, 2 * ox), dtype=torch.complex64, device=dev)
         cptr, keep = nat.int_array(live)
         nat.check(nat.load().oi_spectrum_window(plan.handle, nat.ptr(tens), kind, cin_, cptr, 1,

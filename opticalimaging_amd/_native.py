@@ -45,7 +45,7 @@ class Geometry(ctypes.Structure):
     """Mirror of struct oi_geometry."""
     _fields_ = [(n, c_int) for n in (
         "ny", "nx", "k0r", "wr", "k0c", "wc", "band_r", "band_c", "nchan", "max_batch",
-        "max_groups", "lr", "lc", "rows", "direct", "qr", "qc", "path")]
+        "max_groups", "lr", "lc", "rows", "direct", "qr", "qc", "path", "max_kernels")]
 
 
 class Optics(ctypes.Structure):

@@ -103,8 +103,7 @@ struct oi_plan {
   float* Ic = nullptr;
   float2* Ihat = nullptr;
   float* part = nullptr;    // residual sums of the image-row pass, [B][ny]
-  float2* Bbuf = nullptr;   // two-pass column fields, grown to the largest kernel count used
-  size_t Bcap = 0;
+  float2* Bbuf = nullptr;   // two-pass column fields of max_batch x max_kernels kernels
   size_t bytes = 0;
   std::vector<hipEvent_t> ev;  // slots * (kStages + 1)
   int slots = 0, next = 0, recorded = 0;
@@ -144,6 +143,8 @@ extern "C" int oi_plan_create(oi_geometry* geo, oi_plan** out) {
     return fail(OI_ERR_INVALID, "plan_create: sizes must be positive (nchan <= 3)");
   if (g.wr > g.ny || g.wc > g.nx)
     return fail(OI_ERR_INVALID, "plan_create: spectrum window larger than the grid");
+  if (g.max_kernels < 0) return fail(OI_ERR_INVALID, "plan_create: max_kernels must be >= 0");
+  if (g.max_kernels == 0) g.max_kernels = 256;
   if (g.k0r < -(g.ny / 2) || g.k0r + g.wr - 1 > g.ny - 1 - g.ny / 2 || g.k0c < -(g.nx / 2) ||
       g.k0c + g.wc - 1 > g.nx - 1 - g.nx / 2)
     return fail(OI_ERR_INVALID, "plan_create: window outside the centred index range");
@@ -258,6 +259,8 @@ extern "C" int oi_plan_create(oi_geometry* geo, oi_plan** out) {
       {(void**)&p->Ic, (g.direct && G > 1) ? B * G * (size_t)lr * lc * sizeof(float) : 0},
       {(void**)&p->Ihat, g.path ? B * (size_t)(g.qc + 1) * (2 * g.qr + 1) * sizeof(float2) : 0},
       {(void**)&p->part, g.path ? B * (size_t)g.ny * sizeof(float) : 0},
+      {(void**)&p->Bbuf,
+       g.path == 2 ? B * (size_t)g.max_kernels * C * g.wc * (size_t)lr * sizeof(float2) : 0},
   };
   p->bytes = p->pool.bytes;
   for (auto& w : want) {
@@ -399,23 +402,13 @@ static int coherent_impl(oi_plan* p, const void* spec, int batch, const void* ke
     return fail(OI_ERR_UNSUPPORTED, "socs_residual: plan without the compile-time-size image pass");
   if (batch < 1 || batch > p->g.max_batch) return fail(OI_ERR_INVALID, "batch outside [1, max_batch]");
   if (nkern < 1) return fail(OI_ERR_INVALID, "coherent_image: nkern must be >= 1");
+  if (nkern > p->g.max_kernels)
+    return fail(OI_ERR_INVALID, "coherent_image: nkern exceeds the plan's max_kernels (" +
+                                    std::to_string(p->g.max_kernels) + ")");
   const oi_geometry& g = p->g;
   const int phases = g.lr / g.rows;
   int groups = g.max_groups < nkern ? g.max_groups : nkern;
   if (g.path == 2) {
-    const size_t need = (size_t)batch * nkern * g.nchan * g.wc * g.lr * sizeof(float2);
-    if (need > p->Bcap) {   // first call with this many kernels: grow the column-field buffer
-      if (p->Bbuf) OI_HIP_CHECK(hipFree(p->Bbuf));
-      p->Bbuf = nullptr;
-      p->bytes -= p->Bcap;
-      p->Bcap = 0;
-      if (hipMalloc((void**)&p->Bbuf, need) != hipSuccess) {
-        p->Bbuf = nullptr;
-        return fail(OI_ERR_NOMEM, "coherent_image: column-field buffer allocation failed");
-      }
-      p->Bcap = need;
-      p->bytes += need;
-    }
     FpArgs a = fast_args(p, batch, st);
     a.out = out;
     a.target = target;
@@ -664,6 +657,8 @@ extern "C" int oi_socs_adjoint(oi_plan* p, const float* mask, const float* grad_
   if (p->g.nchan != 1) return fail(OI_ERR_INVALID, "socs_adjoint: the plan must carry one channel (real masks)");
   if (batch < 1 || batch > p->g.max_batch) return fail(OI_ERR_INVALID, "batch outside [1, max_batch]");
   if (nkern < 1) return fail(OI_ERR_INVALID, "socs_adjoint: nkern must be >= 1");
+  if (nkern > p->g.max_kernels)
+    return fail(OI_ERR_INVALID, "socs_adjoint: nkern exceeds the plan's max_kernels");
   hipStream_t st = (hipStream_t)stream;
   const int chan0 = 0;
   const int slots = p->slots;

@@ -7,18 +7,48 @@ c64 array [K][wr][wc] holding each kernel at spectrum index kappa =
 restricted to the kernels' support, so imaging never touches the dense
 (ny, nx, K) array.
 
-``Plan`` owns one library plan per geometry (cached by ``plan_for``);
-``socs_image`` runs ``oi_socs_image`` on the current stream.
+``Plan`` owns one library plan per geometry (cached by ``plan_for`` per
+host thread and HIP stream); ``socs_image`` runs ``oi_socs_image`` on the
+current stream.
 """
 from __future__ import annotations
 
 import collections
 import ctypes
+import threading
+import weakref
 from dataclasses import dataclass, field
 
 import torch
 
 from . import _native as nat
+
+
+class _Attached:
+    """A derived object attached to a tensor, valid while the tensor holds the
+    storage and version it had when the object was attached."""
+    __slots__ = ("obj", "version", "data_ptr")
+
+    def __init__(self, obj, t: torch.Tensor):
+        self.obj = obj
+        self.version = t._version
+        self.data_ptr = t.data_ptr()
+
+
+def attach(t: torch.Tensor, name: str, obj) -> None:
+    """Attach `obj` (derived from t's values) to tensor t under `name`."""
+    setattr(t, name, _Attached(obj, t))
+
+
+def attached(t, name: str):
+    """The object attached to t under `name`, or None when there is none or t
+    was modified in place (or re-pointed) after it was attached."""
+    a = getattr(t, name, None)
+    if not isinstance(a, _Attached) or not isinstance(t, torch.Tensor):
+        return None
+    if t._version != a.version or t.data_ptr() != a.data_ptr:
+        return None
+    return a.obj
 
 
 def centred(idx: torch.Tensor, n: int) -> torch.Tensor:
@@ -99,9 +129,58 @@ class KernelSet:
         return out
 
 
+class _DenseCache:
+    """KernelSets extracted from dense kernel tensors, keyed on the tensor
+    object itself (a weak reference, so a new tensor at a freed tensor's
+    address never matches) and its version (in-place edits invalidate)."""
+
+    def __init__(self, capacity=16):
+        self.capacity = capacity
+        self.items = collections.OrderedDict()
+        self.lock = threading.Lock()
+
+    def get(self, t: torch.Tensor):
+        with self.lock:
+            e = self.items.get(id(t))
+            if e is None:
+                return None
+            ref, version, ptr, ks = e
+            if ref() is not t or t._version != version or t.data_ptr() != ptr:
+                del self.items[id(t)]
+                return None
+            self.items.move_to_end(id(t))
+            return ks
+
+    def put(self, t: torch.Tensor, ks) -> None:
+        with self.lock:
+            self.items[id(t)] = (weakref.ref(t), t._version, t.data_ptr(), ks)
+            while len(self.items) > self.capacity:
+                self.items.popitem(last=False)
+
+
+_DENSE = _DenseCache()
+
+
+def kernel_set_of(kernels) -> "KernelSet":
+    """KernelSet for SOCS kernels given as a KernelSet, as DecomposeTCC_SOCS
+    output (its attached set while unmodified), or as any dense (Ny, Nx, K)
+    complex tensor (torch tensors: extracted once and cached while the same
+    tensor object is unmodified; other arrays: extracted on every call)."""
+    if isinstance(kernels, KernelSet):
+        return kernels
+    ks = attached(kernels, "_oi_kernels")
+    if ks is not None:
+        return ks
+    if isinstance(kernels, torch.Tensor):
+        ks = _DENSE.get(kernels)
+        if ks is None:
+            ks = KernelSet.from_dense(kernels)
+            _DENSE.put(kernels, ks)
+        return ks
+    return KernelSet.from_dense(torch.as_tensor(kernels))
+
+
 PAIR_MAX_K = 512        # larger sets are imaged as they are (the 2K x 2K eigenproblem grows)
-PAIR_MIN_BATCH = 4      # pairing (a 2K x 2K eigenproblem) is first built for a batch this
-                        # large; a one-off image of a few masks keeps the original kernels
 
 
 def real_mask_kernels(ks: KernelSet, rel_tol: float = 1e-10) -> KernelSet:
@@ -164,16 +243,25 @@ class Plan:
     """One oi_plan (twiddles + scratch) for a fixed geometry."""
 
     def __init__(self, ny, nx, k0r, wr, k0c, wc, nchan=1, max_batch=1, max_groups=0,
-                 band_r=0, band_c=0, lr=0, lc=0):
+                 band_r=0, band_c=0, lr=0, lc=0, max_kernels=0):
         lib = nat.load()
         g = nat.Geometry(ny=ny, nx=nx, k0r=k0r, wr=wr, k0c=k0c, wc=wc, band_r=band_r,
                          band_c=band_c, nchan=nchan, max_batch=max_batch, max_groups=max_groups,
-                         lr=lr, lc=lc)
+                         lr=lr, lc=lc, max_kernels=max_kernels)
         h = ctypes.c_void_p()
         nat.check(lib.oi_plan_create(ctypes.byref(g), ctypes.byref(h)))
         self.handle = h
         self.geo = g
-        self.key = (ny, nx, k0r, wr, k0c, wc, nchan, max_batch, max_groups, band_r, band_c, lr, lc)
+        self.key = (ny, nx, k0r, wr, k0c, wc, nchan, max_batch, max_groups, band_r, band_c, lr, lc,
+                    max_kernels)
+
+    @property
+    def max_batch(self) -> int:
+        return int(self.geo.max_batch)
+
+    @property
+    def max_kernels(self) -> int:
+        return int(self.geo.max_kernels)
 
     @property
     def bytes(self) -> int:
@@ -211,21 +299,29 @@ class Plan:
 
 
 class _PlanCache:
-    def __init__(self, capacity=8):
+    """Plans by geometry, one set per (host thread, HIP stream): a plan's
+    scratch is written by every call on it, so two streams (or threads) never
+    share one.  An evicted plan is only dropped from the cache; it is
+    destroyed when the last caller holding it lets it go."""
+
+    def __init__(self, capacity=16):
         self.capacity = capacity
         self.plans = collections.OrderedDict()
+        self.lock = threading.Lock()
 
     def get(self, **kw) -> Plan:
-        key = tuple(sorted(kw.items()))
-        p = self.plans.get(key)
-        if p is not None:
-            self.plans.move_to_end(key)
-            return p
+        owner = (threading.get_ident(), torch.cuda.current_stream().cuda_stream)
+        key = (owner, tuple(sorted(kw.items())))
+        with self.lock:
+            p = self.plans.get(key)
+            if p is not None:
+                self.plans.move_to_end(key)
+                return p
         p = Plan(**kw)
-        self.plans[key] = p
-        while len(self.plans) > self.capacity:
-            _, old = self.plans.popitem(last=False)
-            old.close()
+        with self.lock:
+            self.plans[key] = p
+            while len(self.plans) > self.capacity:
+                self.plans.popitem(last=False)
         return p
 
 
@@ -257,10 +353,49 @@ def field_encoding(field):
 
 
 def plan_for(ks: KernelSet, nchan: int = 1, max_batch: int = 1) -> Plan:
-    """Cached plan for imaging with kernel set `ks`."""
+    """Cached plan for imaging with kernel set `ks` (or any set on its window
+    with at most ks.K kernels, such as ks.for_real_masks()), for the calling
+    thread and the current stream."""
     br, bc = ks.band if ks.band is not None else (0, 0)
     return PLANS.get(ny=ks.ny, nx=ks.nx, k0r=ks.k0r, wr=ks.wr, k0c=ks.k0c, wc=ks.wc, nchan=nchan,
-                     max_batch=max_batch, band_r=min(br, ks.wr), band_c=min(bc, ks.wc))
+                     max_batch=max_batch, band_r=min(br, ks.wr), band_c=min(bc, ks.wc),
+                     max_kernels=ks.K)
+
+
+def _device_tensor(name: str, t, dtype, shape) -> torch.Tensor:
+    """t itself when it is a contiguous `dtype` tensor of `shape` on the
+    current HIP device; ValueError otherwise (the library reads raw pointers)."""
+    if not isinstance(t, torch.Tensor):
+        raise ValueError(f"{name} must be a torch tensor on the HIP device")
+    if not t.is_cuda:
+        raise ValueError(f"{name} must live on the HIP device, got {t.device}")
+    if t.device.index != torch.cuda.current_device():
+        raise ValueError(f"{name} is on {t.device}, the current HIP device is "
+                         f"cuda:{torch.cuda.current_device()}")
+    if t.dtype != dtype:
+        raise ValueError(f"{name} must be {dtype}, got {t.dtype}")
+    if tuple(t.shape) != tuple(shape):
+        raise ValueError(f"{name} must have shape {tuple(shape)}, got {tuple(t.shape)}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+    return t
+
+
+def _check_plan(plan: Plan, ks: KernelSet, nchan: int, B: int) -> None:
+    g = plan.geo
+    if (g.ny, g.nx, g.k0r, g.wr, g.k0c, g.wc) != (ks.ny, ks.nx, ks.k0r, ks.wr, ks.k0c, ks.wc):
+        raise ValueError("plan geometry does not match the kernel set's grid and window")
+    if g.nchan != nchan:
+        raise ValueError(f"plan carries {g.nchan} channels, the call {nchan}")
+    if B > g.max_batch:
+        raise ValueError(f"batch {B} exceeds the plan's max_batch {g.max_batch}")
+
+
+def _real_set(ks: KernelSet, pack_real: bool) -> KernelSet:
+    """The kernel set that images real masks: ks.for_real_masks() (the same
+    image with up to half the kernels, a fixed function of ks) unless
+    pack_real is False.  The choice never depends on earlier calls."""
+    return ks.for_real_masks() if pack_real else ks
 
 
 def socs_image(masks: torch.Tensor, ks: KernelSet, kind: int = nat.OI_FIELD_REAL, cin: int = 1,
@@ -269,24 +404,36 @@ def socs_image(masks: torch.Tensor, ks: KernelSet, kind: int = nat.OI_FIELD_REAL
     """Batched SOCS intensity (CalculateAerialImage_SOCS.py:3-11 per mask).
 
     masks: float32 [B, ny, nx] (kind REAL) or c64 [B, ny, nx, cin] on the
-    device.  Returns float32 [B, ny, nx] in the reference's output layout
-    (after its final fftshift).  Real masks are imaged with
-    ks.for_real_masks() (the same image with up to half the kernels) unless
-    pack_real is False; that set is built on the first call with at least
-    PAIR_MIN_BATCH masks (or by calling ks.for_real_masks()) and reused after."""
+    device, contiguous.  Returns float32 [B, ny, nx] in the reference's output
+    layout (after its final fftshift); `out`, when given, must be exactly
+    that.  Real masks are always imaged with ks.for_real_masks() (the same
+    image with up to half the kernels; built on first use, then reused)
+    unless pack_real is False, so a mask's image does not depend on which
+    calls came before."""
+    if not isinstance(masks, torch.Tensor) or masks.dim() not in (3, 4):
+        raise ValueError("masks must be a (B, ny, nx) or (B, ny, nx, cin) device tensor")
     B = int(masks.shape[0])
     ny, nx = int(masks.shape[1]), int(masks.shape[2])
     if (ny, nx) != (ks.ny, ks.nx):
         raise ValueError(f"mask grid {ny}x{nx} does not match kernel grid {ks.ny}x{ks.nx}")
+    if kind == nat.OI_FIELD_REAL:
+        _device_tensor("masks", masks, torch.float32, (B, ny, nx))
+    elif kind == nat.OI_FIELD_C64_CHLAST:
+        _device_tensor("masks", masks, torch.complex64, (B, ny, nx, cin))
+    else:
+        raise ValueError("unknown field kind")
+    if out is not None:
+        _device_tensor("out", out, torch.float32, (B, ny, nx))
     if B == 0:
         return out if out is not None else torch.empty((0, ny, nx), dtype=torch.float32,
                                                        device=masks.device)
-    if kind == nat.OI_FIELD_REAL and pack_real and (ks._real is not None or B >= PAIR_MIN_BATCH):
-        ks = ks.for_real_masks()
+    if kind == nat.OI_FIELD_REAL:
+        ks = _real_set(ks, pack_real)
     chans = list(chans)
     nchan = len(chans)
     if plan is None:
         plan = plan_for(ks, nchan, max(B, max_batch or 0))
+    _check_plan(plan, ks, nchan, B)
     if out is None:
         out = torch.empty((B, ny, nx), dtype=torch.float32, device=masks.device)
     cptr, keep = nat.int_array(chans)
@@ -303,19 +450,27 @@ def socs_residual(masks: torch.Tensor, ks: KernelSet, target: torch.Tensor, out:
     loss[0] = sum (I - target)^2, fused into the pass that produces I
     (oi_socs_residual).  Returns False, with nothing computed, when the plan
     has no compile-time-size image pass; socs_image then does the work."""
+    if not isinstance(masks, torch.Tensor) or masks.dim() != 3:
+        raise ValueError("masks must be a (B, ny, nx) device tensor")
     B = int(masks.shape[0])
+    shape = (B, ks.ny, ks.nx)
+    if (int(masks.shape[1]), int(masks.shape[2])) != (ks.ny, ks.nx):
+        raise ValueError(f"mask grid does not match kernel grid {ks.ny}x{ks.nx}")
+    _device_tensor("masks", masks, torch.float32, shape)
+    _device_tensor("target", target, torch.float32, shape)
+    _device_tensor("out", out, torch.float32, shape)
+    if loss is not None:
+        if not (loss.is_cuda and loss.dtype == torch.float32 and loss.numel() >= 1
+                and loss.is_contiguous()):
+            raise ValueError("loss must be a contiguous float32 device tensor with one element")
     if B == 0:
         if loss is not None:
             loss.zero_()
         return True
-    if (int(masks.shape[1]), int(masks.shape[2])) != (ks.ny, ks.nx):
-        raise ValueError(f"mask grid does not match kernel grid {ks.ny}x{ks.nx}")
-    if target.shape != masks.shape or out.shape != masks.shape:
-        raise ValueError("target and out must have the masks' shape")
-    if pack_real and (ks._real is not None or B >= PAIR_MIN_BATCH):
-        ks = ks.for_real_masks()
+    ks = _real_set(ks, pack_real)
     if plan is None:
         plan = plan_for(ks, 1, B)
+    _check_plan(plan, ks, 1, B)
     rc = nat.load().oi_socs_residual(plan.handle, nat.ptr(masks), B, nat.ptr(ks.win), ks.K,
                                      nat.ptr(target), nat.ptr(out), nat.ptr(loss), nat.stream())
     if rc == nat.OI_ERR_UNSUPPORTED:
@@ -334,23 +489,28 @@ def socs_adjoint(masks: torch.Tensor, grad_image: torch.Tensor, ks: KernelSet,
     no forward state is reused.  The image is the same function of real masks
     with ks.for_real_masks(), so its gradient is too; that set is used unless
     pack_real is False."""
+    if not isinstance(masks, torch.Tensor) or masks.dim() != 3:
+        raise ValueError("masks must be a (B, ny, nx) device tensor")
     B = int(masks.shape[0])
     ny, nx = int(masks.shape[1]), int(masks.shape[2])
     if (ny, nx) != (ks.ny, ks.nx):
         raise ValueError(f"mask grid {ny}x{nx} does not match kernel grid {ks.ny}x{ks.nx}")
+    if masks.dtype != torch.float32 or not isinstance(grad_image, torch.Tensor) \
+            or grad_image.dtype != torch.float32:
+        raise ValueError("socs_adjoint takes float32 masks and image cotangents")
     if tuple(grad_image.shape) != (B, ny, nx):
         raise ValueError("grad_image must have the shape of masks")
-    if masks.dtype != torch.float32 or grad_image.dtype != torch.float32:
-        raise ValueError("socs_adjoint takes float32 masks and image cotangents")
+    masks = _device_tensor("masks", masks.contiguous(), torch.float32, (B, ny, nx))
+    grad_image = _device_tensor("grad_image", grad_image.contiguous(), torch.float32, (B, ny, nx))
+    if out is not None:
+        _device_tensor("out", out, torch.float32, (B, ny, nx))
     if B == 0:
         return out if out is not None else torch.empty((0, ny, nx), dtype=torch.float32,
                                                        device=masks.device)
-    masks = masks.contiguous()
-    grad_image = grad_image.contiguous()
-    if pack_real and (ks._real is not None or B >= PAIR_MIN_BATCH):
-        ks = ks.for_real_masks()
+    ks = _real_set(ks, pack_real)
     if plan is None:
         plan = plan_for(ks, 1, B)
+    _check_plan(plan, ks, 1, B)
     lib = nat.load()
     nbytes = int(lib.oi_socs_adjoint_work_bytes(plan.handle, B, ks.K))
     work = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=masks.device)

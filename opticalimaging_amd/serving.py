@@ -160,7 +160,7 @@ class GpuExecutor:
             for si in range(streams):
                 p = engine.Plan(ks.ny, ks.nx, ks.k0r, ks.wr, ks.k0c, ks.wc, nchan=1,
                                 max_batch=c.max_batch, band_r=min(br, ks.wr),
-                                band_c=min(bc, ks.wc))
+                                band_c=min(bc, ks.wc), max_kernels=ks.K)
                 self.plans[(c.n, si)] = p
                 self.resident += p.bytes
                 shape = (c.max_batch, c.n, c.n)

@@ -20,6 +20,7 @@ import torch
 import oracle as O
 from oracle.socs_torch import socs_mask_gradient
 from gpu_cases import annular, numerics, projector, rel_err
+from opticalimaging_amd.DecomposeTCC_SOCS import kernels_of
 
 pytestmark = pytest.mark.gpu
 
@@ -73,7 +74,7 @@ def test_adjoint_full_rank_band_plan(gpu):
     num = numerics(order=10 ** 6, precision="fp32")
     tcc, fg = CalculateTCCMatrix(src, [period, period], proj, 0.0, num)
     kern = DecomposeTCC_SOCS(tcc, fg, [n, n], num)
-    ks = kern._oi_kernels
+    ks = kernels_of(kern)
     plan = engine.plan_for(ks, 1, 1)
     assert plan.info()["lr"] < 2 * ks.wr - 1, plan.info()
     masks = _masks(n, n, 1, seed=11)
@@ -97,7 +98,7 @@ def test_adjoint_dot_product_1024(gpu):
     masks_np, period = workloads.socs_1024_masks(batch=2)
     src, proj, num = workloads.socs_1024_optics(order=24, grid_halfwidth=50)
     tcc, fg = CalculateTCCMatrix(src, [period, period], proj, 0.0, num)
-    ks = DecomposeTCC_SOCS(tcc, fg, [1024, 1024], num)._oi_kernels
+    ks = kernels_of(DecomposeTCC_SOCS(tcc, fg, [1024, 1024], num))
     h = torch.as_tensor(masks_np).cuda()
     gen = torch.Generator(device="cuda").manual_seed(5)
     d = 0.5 * torch.randn(h.shape, device="cuda", generator=gen)
@@ -122,7 +123,7 @@ def _c2_kernels_1024():
     tcc, fg = CalculateTCCMatrix(src, [4096.0, 4096.0], proj, 0.0, num)
     kern = DecomposeTCC_SOCS(tcc, fg, [1024, 1024], num)
     del tcc
-    return kern._oi_kernels, src, proj
+    return kernels_of(kern), src, proj
 
 
 def _fields_raw(h, dense, period, NA, lam):

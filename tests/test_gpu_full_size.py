@@ -20,6 +20,7 @@ import torch
 
 import oracle as O
 from gpu_cases import rel_err
+from opticalimaging_amd.DecomposeTCC_SOCS import kernels_of
 
 pytestmark = pytest.mark.gpu
 
@@ -33,7 +34,7 @@ def setup(gpu):
     src, proj, num = workloads.socs_1024_optics(order=24, grid_halfwidth=50)
     tcc, fg = CalculateTCCMatrix(src, [period, period], proj, 0.0, num)
     num.Hopkins_Order = 10 ** 6
-    ks = DecomposeTCC_SOCS(tcc, fg, [N, N], num)._oi_kernels
+    ks = kernels_of(DecomposeTCC_SOCS(tcc, fg, [N, N], num))
     del tcc
     assert ks.K == 120 and ks.band is not None
     plan = engine.plan_for(ks, 1, B)

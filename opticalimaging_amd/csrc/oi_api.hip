@@ -134,10 +134,10 @@ static void plan_free(oi_plan* p) {
   delete p;
 }
 
-extern "C" int oi_plan_create(oi_geometry* geo, oi_plan** out) {
-  if (!geo || !out) return fail(OI_ERR_INVALID, "plan_create: null argument");
-  *out = nullptr;
-  oi_geometry g = *geo;
+// Validates a geometry and fills the library-chosen fields (coarse grid,
+// path, rows, kernel groups, band half-widths); allocates nothing.
+static int plan_geometry(const oi_geometry& in, oi_geometry& g) {
+  g = in;
   if (g.ny <= 0 || g.nx <= 0 || g.wr <= 0 || g.wc <= 0 || g.nchan <= 0 || g.nchan > 3 ||
       g.max_batch <= 0)
     return fail(OI_ERR_INVALID, "plan_create: sizes must be positive (nchan <= 3)");
@@ -175,7 +175,7 @@ extern "C" int oi_plan_create(oi_geometry* geo, oi_plan** out) {
   // Compile-time-size path: both image sizes tabled and a tabled square coarse
   // grid L >= 2 band - 1 in both directions, smaller than the image.
   g.path = 0;
-  if (!g.direct && geo->lr <= 0 && geo->lc <= 0 && fp_image_size_ok(g.ny) && fp_image_size_ok(g.nx)) {
+  if (!g.direct && in.lr <= 0 && in.lc <= 0 && fp_image_size_ok(g.ny) && fp_image_size_ok(g.nx)) {
     const int need = (2 * band_r - 1) > (2 * band_c - 1) ? 2 * band_r - 1 : 2 * band_c - 1;
     const int L = fp_coarse_choose(need);
     if (L > 0 && L < g.ny && L < g.nx && fp_coherent_rows_fit(L) &&
@@ -187,7 +187,7 @@ extern "C" int oi_plan_create(oi_geometry* geo, oi_plan** out) {
   }
   // Two-pass path: coarse grids beyond one workgroup's reach (L = 256, 512)
   // when the window folds onto them without collisions.
-  if (g.path == 0 && !g.direct && geo->lr <= 0 && geo->lc <= 0) {
+  if (g.path == 0 && !g.direct && in.lr <= 0 && in.lc <= 0) {
     const int need = (2 * band_r - 1) > (2 * band_c - 1) ? 2 * band_r - 1 : 2 * band_c - 1;
     // coarse grids past one workgroup's reach, and those the one-pass kernel
     // does not serve when the two-pass table has them (144)
@@ -231,13 +231,43 @@ extern "C" int oi_plan_create(oi_geometry* geo, oi_plan** out) {
     int gr = (target + phases * g.max_batch - 1) / (phases * g.max_batch);
     g.max_groups = gr < 1 ? 1 : (gr > 64 ? 64 : gr);
   }
+  return OI_OK;
+}
 
+// Device scratch of a plan for max_batch images: byte size of each buffer.
+struct ScratchLayout {
+  size_t T, spec, R, U, Ic, Ihat, part, Bbuf;
+  size_t total() const { return T + spec + R + U + Ic + Ihat + part + Bbuf; }
+};
+
+static ScratchLayout scratch_layout(const oi_geometry& g, size_t B) {
+  const size_t C = (size_t)g.nchan, G = (size_t)g.max_groups;
+  const size_t lr = (size_t)g.lr, lc = (size_t)g.lc;
+  ScratchLayout s;
+  s.T = B * C * g.wc * g.ny * sizeof(float2);
+  s.spec = B * C * g.wr * g.wc * sizeof(float2);
+  s.R = g.direct ? 0 : B * G * lr * (size_t)(g.qc + 1) * sizeof(float2);
+  s.U = g.direct ? 0 : B * (size_t)g.ny * (g.qc + 1) * sizeof(float2);
+  s.Ic = (g.direct && G > 1) ? B * G * lr * lc * sizeof(float) : 0;
+  s.Ihat = g.path ? B * (size_t)(g.qc + 1) * (2 * g.qr + 1) * sizeof(float2) : 0;
+  s.part = g.path ? B * (size_t)g.ny * sizeof(float) : 0;
+  s.Bbuf = g.path == 2 ? B * (size_t)g.max_kernels * C * g.wc * lr * sizeof(float2) : 0;
+  return s;
+}
+
+extern "C" int oi_plan_create(oi_geometry* geo, oi_plan** out) {
+  if (!geo || !out) return fail(OI_ERR_INVALID, "plan_create: null argument");
+  *out = nullptr;
+  oi_geometry g;
+  int rc = plan_geometry(*geo, g);
+  if (rc) return rc;
+  const int lr = g.lr, lc = g.lc, rows = g.rows;
   oi_plan* p = new (std::nothrow) oi_plan();
   if (!p) return fail(OI_ERR_NOMEM, "plan_create: host allocation failed");
   p->g = g;
   int sizes[5] = {g.nx, g.ny, lr, lc, rows};
   FftDesc d[5];
-  int rc = make_twiddles(sizes, 5, d, p->pool);
+  rc = make_twiddles(sizes, 5, d, p->pool);
   if (rc) {
     plan_free(p);
     return rc;
@@ -247,20 +277,14 @@ extern "C" int oi_plan_create(oi_geometry* geo, oi_plan** out) {
   p->flr = d[2];
   p->flc = d[3];
   p->frows = d[4];
-  const size_t B = (size_t)g.max_batch, C = (size_t)g.nchan, G = (size_t)g.max_groups;
+  const ScratchLayout sl = scratch_layout(g, (size_t)g.max_batch);
   struct Want {
     void** ptr;
     size_t bytes;
   } want[] = {
-      {(void**)&p->T, B * C * g.wc * g.ny * sizeof(float2)},
-      {(void**)&p->spec, B * C * g.wr * g.wc * sizeof(float2)},
-      {(void**)&p->R, g.direct ? 0 : B * G * lr * (size_t)(g.qc + 1) * sizeof(float2)},
-      {(void**)&p->U, g.direct ? 0 : B * (size_t)g.ny * (g.qc + 1) * sizeof(float2)},
-      {(void**)&p->Ic, (g.direct && G > 1) ? B * G * (size_t)lr * lc * sizeof(float) : 0},
-      {(void**)&p->Ihat, g.path ? B * (size_t)(g.qc + 1) * (2 * g.qr + 1) * sizeof(float2) : 0},
-      {(void**)&p->part, g.path ? B * (size_t)g.ny * sizeof(float) : 0},
-      {(void**)&p->Bbuf,
-       g.path == 2 ? B * (size_t)g.max_kernels * C * g.wc * (size_t)lr * sizeof(float2) : 0},
+      {(void**)&p->T, sl.T},       {(void**)&p->spec, sl.spec}, {(void**)&p->R, sl.R},
+      {(void**)&p->U, sl.U},       {(void**)&p->Ic, sl.Ic},     {(void**)&p->Ihat, sl.Ihat},
+      {(void**)&p->part, sl.part}, {(void**)&p->Bbuf, sl.Bbuf},
   };
   p->bytes = p->pool.bytes;
   for (auto& w : want) {

@@ -59,16 +59,22 @@ class Request:
 class Completed:
     rid: int
     n: int
-    image: object           # float32 (n, n) on the device
+    image: object           # float32 (n, n) on the device (None unless status is "ok")
     arrival: float
     admitted: float
     launched: float
     done: float
     batch: int
+    status: str = "ok"      # "ok" | "error" | "cancelled"
+    error: str | None = None
 
     @property
     def latency(self) -> float:
         return self.done - self.arrival
+
+    @property
+    def ok(self) -> bool:
+        return self.status == "ok"
 
 
 @dataclass
@@ -223,7 +229,18 @@ class _Entry:
 
 class Server:
     """Continuous batching under an HBM budget (bytes, including the resident
-    kernel sets and plan scratch reported by the executor)."""
+    kernel sets and plan scratch reported by the executor).
+
+    Failures stay with their request: a request that cannot be served (no
+    size class for its clip, a malformed mask or level table, more bytes
+    than the budget can ever hold) completes at once with status "error";
+    a batch whose launch raises is retried one request at a time, so only
+    the requests that fail alone are reported as errors; a batch whose
+    completion check raises fails its own requests only.  ``cancel(rid)``
+    withdraws a request: before launch it never reaches the device and its
+    bytes are released; once launched its result is discarded when the
+    batch completes.  Cancelled and failed requests appear in ``completed``
+    with their status and no image."""
 
     def __init__(self, classes, executor, hbm_budget: float = 8e9, keep_images: bool = True):
         self.classes = {c.n: c for c in classes} if not isinstance(classes, dict) else classes
@@ -243,35 +260,134 @@ class Server:
         self.running = []
         self.free_slots = list(range(executor.slots))
         self.completed = []
+        self.cancelled = set()       # rids withdrawn before they completed
         self.keep_images = keep_images      # False: Completed.image is None (timing runs)
+
+    # -- per-request checks ---------------------------------------------------
+    def problem(self, r: Request) -> str | None:
+        """Why request r cannot be served, or None."""
+        mask = r.mask
+        shape = tuple(getattr(mask, "shape", ()))
+        if len(shape) != 2 or shape[0] != shape[1]:
+            return f"mask must be a square (n, n) array, got shape {shape}"
+        c = self.classes.get(int(shape[0]))
+        if c is None:
+            return f"no size class for {shape[0]}^2 requests"
+        dt = str(getattr(mask, "dtype", ""))
+        if r.coded:
+            if "uint8" not in dt:
+                return f"coded masks are uint8, got {dt}"
+            if not 1 <= len(r.levels) <= 256:
+                return "a level table holds 1 to 256 values"
+        elif "float32" not in dt:
+            return f"masks are float32 (or uint8 codes with levels), got {dt}"
+        if c.bytes_for(r) > self.room:
+            return f"request {r.rid} ({c.bytes_for(r)} B) does not fit in the HBM budget"
+        return None
+
+    def _fail(self, r: Request, now, why, admitted=None, launched=None, status="error"):
+        self.completed.append(Completed(r.rid, int(getattr(r.mask, "shape", (0,))[0]), None,
+                                        r.arrival, now if admitted is None else admitted,
+                                        now if launched is None else launched, now, 0,
+                                        status=status, error=why))
+
+    def cancel(self, rid: int) -> None:
+        """Withdraw request rid (no effect once it has completed)."""
+        self.cancelled.add(rid)
 
     # -- one pass of the loop -------------------------------------------------
     def _complete(self, now):
         still = []
         for b in self.running:
-            if self.ex.done(b.handle):
-                outs = self.ex.outputs(b.handle) if self.keep_images else [None] * len(b.reqs)
-                for e, img in zip(b.reqs, outs):
-                    self.completed.append(Completed(e.req.rid, e.req.n, img, e.req.arrival,
-                                                    e.admitted, b.launched, now, len(b.reqs)))
-                    self.in_use -= e.nbytes
-                self.free_slots.append(b.slot)
+            try:
+                finished = self.ex.done(b.handle)
+                outs = None
+                if finished and self.keep_images:
+                    outs = self.ex.outputs(b.handle)
+            except Exception as exc:          # the batch's own requests fail, nothing else
+                finished, outs = True, None
+                err = f"{type(exc).__name__}: {exc}"
             else:
+                err = None
+            if not finished:
                 still.append(b)
+                continue
+            outs = outs if outs is not None else [None] * len(b.reqs)
+            for e, img in zip(b.reqs, outs):
+                rid = e.req.rid
+                if rid in self.cancelled:
+                    status, img, why = "cancelled", None, "cancelled after launch"
+                    self.cancelled.discard(rid)
+                elif err is not None:
+                    status, img, why = "error", None, err
+                else:
+                    status, why = "ok", None
+                self.completed.append(Completed(rid, e.req.n, img, e.req.arrival, e.admitted,
+                                                b.launched, now, len(b.reqs), status=status,
+                                                error=why))
+                self.in_use -= e.nbytes
+            self.free_slots.append(b.slot)
         self.running = still
+
+    def _drop_cancelled(self, now):
+        if not self.cancelled:
+            return
+        keep = collections.deque()
+        for r in self.waiting:
+            if r.rid in self.cancelled:
+                self.cancelled.discard(r.rid)
+                self._fail(r, now, "cancelled before admission", status="cancelled")
+            else:
+                keep.append(r)
+        self.waiting = keep
+        for n, q in self.pending.items():
+            keep = collections.deque()
+            for e in q:
+                if e.req.rid in self.cancelled:
+                    self.cancelled.discard(e.req.rid)
+                    self.in_use -= e.nbytes
+                    self._fail(e.req, now, "cancelled before launch", admitted=e.admitted,
+                               status="cancelled")
+                else:
+                    keep.append(e)
+            self.pending[n] = keep
 
     def _admit(self, now):
         while self.waiting:
             r = self.waiting[0]
+            why = self.problem(r)
+            if why is not None:
+                self.waiting.popleft()
+                self._fail(r, now, why)
+                continue
             nb = self.classes[r.n].bytes_for(r)
-            if nb > self.room:
-                raise ValueError(f"request {r.rid} ({nb} B) does not fit in the HBM budget")
             if self.resident + self.in_use + nb > self.budget:
                 break
             self.waiting.popleft()
             self.in_use += nb
             self.peak_in_use = max(self.peak_in_use, self.in_use)
             self.pending[r.n].append(_Entry(r, now, nb))
+
+    def _try_launch(self, c, slot, entries, now):
+        """Launch `entries` on `slot`; on failure retry them one at a time and
+        report those that fail alone.  Returns True when the slot was used."""
+        try:
+            handle = self.ex.launch(c, slot, [e.req for e in entries])
+        except Exception as exc:
+            if len(entries) == 1:
+                e = entries[0]
+                self.in_use -= e.nbytes
+                self._fail(e.req, now, f"{type(exc).__name__}: {exc}", admitted=e.admitted)
+                return False
+            used = False
+            for i, e in enumerate(entries):
+                if used:                      # the slot is busy: the rest wait for the next pass
+                    self.pending[c.n].extendleft(reversed(entries[i:]))
+                    break
+                used = self._try_launch(c, slot, [e], now)
+            return used
+        self.running.append(_Batch(c, slot, entries, now, handle))
+        return True
 
     def _launch(self, now):
         while self.free_slots:
@@ -283,37 +399,38 @@ class Server:
             q = self.pending[n]
             take = [q.popleft() for _ in range(min(c.max_batch, len(q)))]
             slot = self.free_slots.pop(0)
-            handle = self.ex.launch(c, slot, [e.req for e in take])
-            self.running.append(_Batch(c, slot, take, now, handle))
+            if not self._try_launch(c, slot, take, now):
+                self.free_slots.insert(0, slot)
 
     def step(self, now):
         self._complete(now)
+        self._drop_cancelled(now)
         self._admit(now)
         self._launch(now)
 
     # -- driving a request stream ---------------------------------------------
-    def run(self, requests, clock=None, idle_sleep: float = 2e-5):
+    def run(self, requests, clock=None, idle_sleep: float = 2e-5, cancel_at=None):
         """Serve `requests` (sorted by arrival, seconds from the start) in real
-        time; returns the Completed records in completion order."""
-        for r in requests:
-            if self.classes.get(r.n) is None:
-                raise ValueError(f"no size class for {r.n}^2 requests")
-            if self.classes[r.n].bytes_for(r) > self.room:
-                raise ValueError(f"request {r.rid} does not fit in the HBM budget")
+        time; returns the Completed records in completion order.
+        cancel_at: optional {rid: seconds} at which a request is withdrawn."""
         clock = clock or time.perf_counter
+        cancels = sorted((t, rid) for rid, t in (cancel_at or {}).items())
         # the cyclic garbage collector is paused while serving: a full
         # collection over the run's records stalls the loop for milliseconds
         gc_on = gc.isenabled()
         gc.disable()
         try:
             t0 = clock()
-            i = 0
+            i = j = 0
             total = len(requests)
             while len(self.completed) < total:
                 now = clock() - t0
                 while i < total and requests[i].arrival <= now:
                     self.waiting.append(requests[i])
                     i += 1
+                while j < len(cancels) and cancels[j][0] <= now:
+                    self.cancel(cancels[j][1])
+                    j += 1
                 before = (len(self.completed), len(self.running))
                 self.step(now)
                 if (len(self.completed), len(self.running)) == before and idle_sleep:
@@ -325,6 +442,9 @@ class Server:
 
 
 def latency_stats(done):
+    """p50/p99/mean/max latency (ms), images/s and mean batch over the
+    requests that completed with status "ok"."""
+    done = [c for c in done if getattr(c, "status", "ok") == "ok"]
     lat = np.array([c.latency for c in done]) * 1e3
     span = max(c.done for c in done) - min(c.arrival for c in done)
     return {"requests": len(done), "p50_ms": float(np.percentile(lat, 50)),

@@ -158,7 +158,94 @@ def test_coded_requests_count_their_code_bytes():
     inflight_max = _drive(srv, ex, reqs)
     assert inflight_max <= 2 and srv.peak_in_use <= 2 * per
     assert srv.in_use == 0 and len(srv.completed) == 9
-    # a coded clip that cannot fit even alone is refused before serving starts
+    # a coded clip that cannot fit even alone fails alone, with a reason
     tight = Server([cls], SimExecutor(clock), hbm_budget=cls.request_bytes + 10)
-    with pytest.raises(ValueError):
-        tight.run([coded])
+    done = tight.run([coded])
+    assert len(done) == 1 and done[0].status == "error" and "budget" in done[0].error
+    assert tight.in_use == 0
+
+
+class FlakyExecutor(SimExecutor):
+    """Raises on launch whenever a batch holds one of the `bad` request ids."""
+
+    def __init__(self, clock, bad, **kw):
+        super().__init__(clock, **kw)
+        self.bad = set(bad)
+
+    def launch(self, cls, slot, reqs):
+        if any(r.rid in self.bad for r in reqs):
+            raise RuntimeError("device rejected the batch")
+        return super().launch(cls, slot, reqs)
+
+
+def test_failures_stay_with_their_request():
+    """Malformed requests, requests of an unknown size and requests whose
+    launch fails complete with status "error"; every other request of the
+    stream, including the failing ones' batchmates, is served."""
+    clock = [0.0]
+    ex = FlakyExecutor(clock, bad={7, 30}, slots=2)
+    srv = Server(_classes(max_batch=4), ex, hbm_budget=1e9)
+    reqs = _requests(60, rate=20000, seed=3)
+    reqs[3] = Request(3, np.zeros((300, 300), np.float32), reqs[3].arrival)        # no class
+    reqs[11] = Request(11, np.zeros((256, 512), np.float32), reqs[11].arrival)     # not square
+    reqs[12] = Request(12, np.zeros((256, 256), np.float64), reqs[12].arrival)     # wrong dtype
+    reqs[13] = Request(13, np.zeros((256, 256), np.uint8), reqs[13].arrival, levels=())
+    _drive(srv, ex, reqs)
+    by = {c.rid: c for c in srv.completed}
+    assert sorted(by) == list(range(60))
+    bad = {3, 7, 11, 12, 13, 30}
+    for rid, c in by.items():
+        if rid in bad:
+            assert c.status == "error" and c.image is None and c.error, rid
+        else:
+            assert c.status == "ok" and c.image == ("img", rid), rid
+    # the failing launches' batchmates were relaunched and served
+    launched = [rid for *_, rids in ex.launches for rid in rids]
+    assert 7 not in launched and 30 not in launched
+    assert srv.in_use == 0 and not srv.running
+    st = latency_stats(srv.completed)
+    assert st["requests"] == 60 - len(bad)
+
+
+def test_cancellation_before_and_after_launch():
+    clock = [0.0]
+    ex = SimExecutor(clock, slots=1, base=1e-3, per_image=0.0)
+    cls = [SizeClass(256, None, 2)]
+    srv = Server(cls, ex, hbm_budget=1e9)
+    reqs = [Request(i, np.zeros((256, 256), np.float32), 0.0) for i in range(6)]
+    for r in reqs:
+        srv.waiting.append(r)
+    srv.step(0.0)                           # admits all six, launches 0 and 1
+    assert [e.req.rid for e in srv.running[0].reqs] == [0, 1]
+    srv.cancel(1)                           # running: result discarded at completion
+    srv.cancel(4)                           # pending: never launched, bytes released
+    srv.cancel(99)                          # unknown: no effect
+    in_use = srv.in_use
+    srv.step(1e-4)
+    assert srv.in_use == in_use - cls[0].request_bytes
+    t = 1e-4
+    while len(srv.completed) < 6:
+        t += 2e-4
+        clock[0] = t
+        srv.step(t)
+    by = {c.rid: c for c in srv.completed}
+    assert by[1].status == "cancelled" and by[1].image is None
+    assert by[4].status == "cancelled" and by[4].batch == 0
+    assert all(by[i].status == "ok" for i in (0, 2, 3, 5))
+    launched = [rid for *_, rids in ex.launches for rid in rids]
+    assert 4 not in launched and srv.in_use == 0
+
+
+def test_run_cancel_at():
+    """run(cancel_at=...) withdraws a request that is still waiting."""
+    clock = [0.0]
+    ex = SimExecutor(clock, slots=1, base=5e-3, per_image=0.0)
+    srv = Server([SizeClass(256, None, 1)], ex, hbm_budget=1e9)
+    reqs = [Request(i, np.zeros((256, 256), np.float32), 0.0) for i in range(3)]
+
+    def tick():
+        clock[0] += 1e-3
+        return clock[0]
+    done = srv.run(reqs, clock=tick, idle_sleep=0.0, cancel_at={2: 0.0})
+    by = {c.rid: c.status for c in done}
+    assert by == {0: "ok", 1: "ok", 2: "cancelled"}

@@ -1,0 +1,433 @@
+"""Float64 numpy restatement of the reference's RCWA near-field producer
+(rcwa.py:4-872).  TEST INFRASTRUCTURE ONLY: imported by tests/ as the
+checker of opticalimaging_amd.rcwa; never by the package.
+
+Same algorithm as the reference, in complex128 throughout:
+  * k-vectors and the free-space / input / output mode matrices
+    (rcwa.py:349-433, get_V 865-872);
+  * convolution matrix of a permittivity grid (rcwa.py:435-450);
+  * layer eigenmodes from eig(PQ) (rcwa.py:452-511);
+  * layer S-matrix in the free-space-gap form, stable-inverse branch
+    (rcwa.py:513-590);
+  * Redheffer star product and its binary power (rcwa.py:695-736);
+  * global S-matrix, plain and with a repeated block (rcwa.py:159-223);
+  * plane-wave / Fourier source, field synthesis on an x-y grid and Floquet
+    amplitudes (rcwa.py:225-338);
+  * the second-order eigen-perturbation of a defect (rcwa.py:592-693).
+The reference's matrix-exponential route (rcwa.py:738-863, `fast=True`) is
+restated through scipy.linalg.expm (a Pade scaling-and-squaring method)
+instead of the reference's degree-18 polynomial, then converted to the
+layer S-matrix with the same transfer-to-scattering formulas.
+
+Deliberate difference: the output layer's mode matrix is get_V(eps_out)
+(the evident intent of rcwa.py:407-431, which assigns self.Vi and then reads
+an undefined self.Vo).
+
+Pins (tests/test_oracle_rcwa.py): energy conservation R + T = 1 for lossless
+gratings; the Airy closed form of a homogeneous slab at normal and oblique
+incidence; eig route == matrix-exponential route per layer; the star product
+of a layer split in two equals the whole layer; the defect perturbation
+approaches the exact layer as the defect shrinks; transversality of the
+synthesised fields; and the reference's own outputs on small gratings
+(tests/golden/reference_rcwa.npz, written by tests/golden/make_golden_rcwa.py).
+"""
+from __future__ import annotations
+
+import numpy as np
+import scipy.linalg
+
+
+def _sqrt_upper(z):
+    """sqrt with the branch used throughout rcwa.py: conj when Im < 0."""
+    k = np.sqrt(z.astype(np.complex128))
+    return np.where(np.imag(k) < 0, np.conj(k), k)
+
+
+class RcwaOracle:
+    """rcwa.py:4-872 in float64 (reference method names without leading
+    underscores where they are internal)."""
+
+    def __init__(self, freq, order, L):
+        # rcwa.py:6-77
+        self.freq = float(freq)
+        self.omega = 2 * np.pi * self.freq
+        self.order = [int(order[0]), int(order[1])]
+        self.order_x = np.arange(-self.order[0], self.order[0] + 1)
+        self.order_y = np.arange(-self.order[1], self.order[1] + 1)
+        self.order_N = len(self.order_x) * len(self.order_y)
+        self.L = [float(L[0]), float(L[1])]
+        self.Gx_norm = 1.0 / (self.L[0] * self.freq)
+        self.Gy_norm = 1.0 / (self.L[1] * self.freq)
+        self.eps_in = 1.0 + 0j
+        self.eps_out = 1.0 + 0j
+        self.has_in = False
+        self.has_out = False
+        self.layer_N = 0
+        self.thickness, self.eps_conv, self.P, self.Q, self.PQ = [], [], [], [], []
+        self.kz_norm, self.E_eigvec, self.H_eigvec = [], [], []
+        self.S11, self.S21, self.S12, self.S22 = [], [], [], []
+
+    # ------------------------------------------------------------ set-up
+    def add_input_layer(self, eps=1.0):
+        self.eps_in = complex(eps)
+        self.has_in = True
+
+    def add_output_layer(self, eps=1.0):
+        self.eps_out = complex(eps)
+        self.has_out = True
+
+    def set_incident_angle(self, inc_ang, azi_ang, angle_layer="input"):
+        # rcwa.py:103-122
+        self.inc_ang = float(inc_ang)
+        self.azi_ang = float(azi_ang)
+        self.angle_layer = "output" if angle_layer in ("o", "out", "output") else "input"
+        self.kvectors()
+
+    def kvectors(self):
+        # rcwa.py:349-433
+        eps = self.eps_in if self.angle_layer == "input" else self.eps_out
+        n = np.real(np.sqrt(complex(eps)))
+        self.kx0_norm = n * np.sin(self.inc_ang) * np.cos(self.azi_ang)
+        self.ky0_norm = n * np.sin(self.inc_ang) * np.sin(self.azi_ang)
+        self.kx_norm = self.kx0_norm + self.order_x * self.Gx_norm
+        self.ky_norm = self.ky0_norm + self.order_y * self.Gy_norm
+        gx, gy = np.meshgrid(self.kx_norm, self.ky_norm, indexing="ij")
+        self.Kx_dn = gx.reshape(-1).astype(np.complex128)
+        self.Ky_dn = gy.reshape(-1).astype(np.complex128)
+        self.Kx = np.diag(self.Kx_dn)
+        self.Ky = np.diag(self.Ky_dn)
+        self.Vf, _ = self.get_V(1.0)
+        n2 = 2 * self.order_N
+        if self.has_in:
+            Vi, _ = self.get_V(self.eps_in)
+            a, b = self.Vf + Vi, self.Vf - Vi
+            S11 = 2 * np.linalg.solve(a, Vi)
+            S12 = np.linalg.solve(a, b)
+            self.Sin = [S11, -S12, S12, 2 * np.linalg.solve(a, self.Vf)]
+        if self.has_out:
+            Vo, _ = self.get_V(self.eps_out)
+            a, b = self.Vf + Vo, self.Vf - Vo
+            S12 = -np.linalg.solve(a, b)
+            self.Sout = [2 * np.linalg.solve(a, self.Vf), S12, S12, 2 * np.linalg.solve(a, Vo)]
+        self.I2 = np.eye(n2, dtype=np.complex128)
+
+    def get_V(self, eps):
+        # rcwa.py:865-872
+        kz = _sqrt_upper(complex(eps) - self.Kx_dn ** 2 - self.Ky_dn ** 2)
+        kzi = np.diag(1.0 / kz)
+        KxKyKzi = self.Kx * self.Ky @ kzi
+        A = np.diag(kz) + self.Kx ** 2 @ kzi
+        B = np.diag(kz) + self.Ky ** 2 @ kzi
+        V = np.block([[-KxKyKzi, -B], [A, KxKyKzi]])
+        return V, kz
+
+    def material_conv(self, material):
+        # rcwa.py:435-450: E[i][j] = fft2(eps)/N at (ox_i - ox_j, oy_i - oy_j)
+        m = np.asarray(material, dtype=np.complex128)
+        F = np.fft.fft2(m) / m.size
+        gx, gy = np.meshgrid(self.order_x, self.order_y, indexing="ij")
+        ox, oy = gx.reshape(-1), gy.reshape(-1)
+        return F[(ox[:, None] - ox[None, :]) % m.shape[0], (oy[:, None] - oy[None, :]) % m.shape[1]]
+
+    @staticmethod
+    def _is_homogeneous(eps):
+        a = np.asarray(eps)
+        return a.ndim == 0 or (a.ndim == 1 and a.shape[0] == 1)
+
+    def pq(self, E):
+        """P, Q of a layer with convolution matrix E (rcwa.py:470-497)."""
+        N = self.order_N
+        EiKx = np.linalg.solve(E, self.Kx)
+        EiKy = np.linalg.solve(E, self.Ky)
+        I = np.eye(N, dtype=np.complex128)
+        KxKx, KxKy, KyKy = self.Kx ** 2, self.Kx * self.Ky, self.Ky ** 2
+        P = np.block([[self.Kx @ EiKy, I - self.Kx @ EiKx], [self.Ky @ EiKy - I, -self.Ky @ EiKx]])
+        Q = np.block([[-KxKy, KxKx - E], [E - KyKy, KxKy]])
+        return P, Q
+
+    def add_layer(self, thickness, eps=1.0, route="eig"):
+        # rcwa.py:124-151
+        N = self.order_N
+        if self._is_homogeneous(eps):
+            e = complex(np.asarray(eps).reshape(-1)[0])
+            E = e * np.eye(N, dtype=np.complex128)
+        else:
+            e = None
+            E = self.material_conv(eps)
+        self.eps_conv.append(E)
+        self.layer_N += 1
+        self.thickness.append(float(thickness))
+        if route == "expm":
+            self._layer_expm(E, float(thickness))
+            return
+        if e is not None:
+            self._eigen_homogeneous(e)
+        else:
+            self._eigen(E)
+        self._solve_layer_smatrix()
+
+    def _eigen_homogeneous(self, eps):
+        # rcwa.py:452-467
+        V, kz = self.get_V(eps)
+        N = self.order_N
+        I = np.eye(N, dtype=np.complex128)
+        KxKx, KxKy, KyKy = self.Kx ** 2, self.Kx * self.Ky, self.Ky ** 2
+        # rcwa.py:458 stacks these blocks in the transposed arrangement of the
+        # inhomogeneous P (they are only read by the defect perturbation)
+        P = np.block([[KxKy / eps, KyKy / eps - I], [I - KxKx / eps, -KxKy / eps]])
+        Q = -eps * P
+        self.P.append(P)
+        self.Q.append(Q)
+        self.PQ.append(np.kron(np.eye(2), np.diag(eps - self.Kx_dn ** 2 - self.Ky_dn ** 2)))
+        self.kz_norm.append(np.concatenate([kz, kz]))
+        self.E_eigvec.append(np.eye(2 * N, dtype=np.complex128))
+        self.H_eigvec.append(V)
+
+    def _eigen(self, E):
+        # rcwa.py:469-511
+        P, Q = self.pq(E)
+        PQ = P @ Q
+        lam, W = np.linalg.eig(PQ)
+        kz = np.sqrt(lam.astype(np.complex128))
+        kz = np.where(np.imag(kz) < 0, -kz, kz)
+        self.P.append(P)
+        self.Q.append(Q)
+        self.PQ.append(PQ)
+        self.kz_norm.append(kz)
+        self.E_eigvec.append(W)
+        kzi = np.where(kz == 0, 0, 1.0 / np.where(kz == 0, 1, kz))
+        self.H_eigvec.append(Q @ W @ np.diag(kzi))
+
+    def _smatrix_from_modes(self, W, V, kz, d):
+        """Layer S-matrix between free-space gaps, rcwa.py:513-545 (the branch
+        where A is invertible)."""
+        phase = np.diag(np.exp(1j * self.omega * kz * d))
+        V0iV = -self.Vf @ V
+        A = W + V0iV
+        B = (W - V0iV) @ phase
+        BAi = np.linalg.solve(A.T, B.T).T
+        T = (A - BAi @ B).T
+        WX = W @ phase
+        WXC1 = 2 * np.linalg.solve(T, WX.T).T
+        WC1 = 2 * np.linalg.solve(T, W.T).T
+        WXC2 = -WXC1 @ BAi
+        WC2 = -WC1 @ BAi
+        return WXC1 + WC2, WXC2 + WC1 - self.I2
+
+    def _solve_layer_smatrix(self):
+        S11, S12 = self._smatrix_from_modes(self.E_eigvec[-1], self.H_eigvec[-1], self.kz_norm[-1],
+                                            self.thickness[-1])
+        self.S11.append(S11)
+        self.S12.append(S12)
+        self.S21.append(S12)
+        self.S22.append(S11)
+
+    def _layer_expm(self, E, d):
+        """Layer S-matrix from the transfer matrix exp(k0 d [[0, -P], [Q, 0]])
+        (rcwa.py:738-845 with scipy's expm), split into 2^n blocks whose
+        S-matrices are star-multiplied (rcwa.py:832-836)."""
+        P, Q = self.pq(E)
+        n2 = 2 * self.order_N
+        thres = 2.0 * min(self.L[0] / max(self.order[0], 1), self.L[1] / max(self.order[1], 1))
+        n = int(np.ceil(np.log2(d / thres))) if d > thres else 0
+        db = d / 2 ** n
+        k0d = self.omega * db
+        Om = np.block([[np.zeros((n2, n2)), -k0d * P], [k0d * Q, np.zeros((n2, n2))]])
+        X = scipy.linalg.expm(Om)
+        e11, e12, e21, e22 = X[:n2, :n2], X[:n2, n2:], X[n2:, :n2], X[n2:, n2:]
+        Ev = e11 + 1j * self.Vf @ e21
+        F = (1j * e12 - self.Vf @ e22) @ self.Vf
+        T22, T21 = 0.5 * (Ev + F), 0.5 * (Ev - F)
+        S11 = np.linalg.inv(T22)
+        S12 = -S11 @ T21
+        for _ in range(n):
+            R = self.I2 - S12 @ S12
+            S12, S11 = S12 + S11 @ np.linalg.solve(R, S12 @ S11), S11 @ np.linalg.solve(R, S11)
+        self.P.append(P)
+        self.Q.append(Q)
+        self.S11.append(S11)
+        self.S12.append(S12)
+        self.S21.append(S12)
+        self.S22.append(S11)
+
+    # ----------------------------------------------------------- defects
+    def add_defect(self, layer_num=0, deps=None, use_2ndorder=True):
+        """Second-order eigen-perturbation of layer layer_num by a permittivity
+        change deps (rcwa.py:154-157, 592-693)."""
+        Ed = self.material_conv(deps)
+        E0 = self.eps_conv[layer_num]
+        PQ0, Q0, W0, Kz0 = self.PQ[layer_num], self.Q[layer_num], self.E_eigvec[layer_num], \
+            self.kz_norm[layer_num]
+        Kx, Ky = self.Kx, self.Ky
+        O = np.zeros_like(E0)
+        E0iEd = np.linalg.solve(E0, Ed)
+        E0iKx = np.linalg.solve(E0, Kx)
+        E0iKy = np.linalg.solve(E0, Ky)
+        d11 = Ed + Kx @ E0iEd @ E0iKx @ E0 - Kx @ E0iKx @ Ed
+        d12 = Kx @ E0iEd @ E0iKy @ E0 - Kx @ E0iKy @ Ed
+        d21 = Ky @ E0iEd @ E0iKx @ E0 - Ky @ E0iKx @ Ed
+        d22 = Ed + Ky @ E0iEd @ E0iKy @ E0 - Ky @ E0iKy @ Ed
+        PQd = np.block([[d11, d12], [d21, d22]])
+        Q = Q0 + np.block([[O, -Ed], [Ed, O]])
+        Anorm = np.linalg.norm(PQ0)
+        eig_tol = 1.0 / Anorm
+        Kz0sq = Kz0 ** 2
+        phi = Kz0sq[:, None] - Kz0sq[None, :]
+        phi = np.where(np.abs(phi) < eig_tol, 0, 1.0 / np.where(np.abs(phi) < eig_tol, 1, phi))
+        WdW = np.linalg.solve(W0, PQd @ W0)
+        ld = np.diagonal(WdW)
+        half = np.where(Kz0 == 0, 0, 0.5 / np.where(Kz0 == 0, 1, Kz0))
+        Kzd = ld * half
+        Wd = -W0 @ (phi * WdW)
+        kz = Kz0 + Kzd
+        W = W0 + Wd
+        if use_2ndorder:
+            WdWd = np.linalg.solve(W0, PQd @ Wd)
+            ldd = np.diagonal(WdWd)
+            kz = kz + ldd * half
+            # rcwa.py:638 writes Wd @ ld with ld a vector: a matrix-vector
+            # product broadcast along the rows of PQd @ Wd, restated as such
+            second = np.linalg.solve(W0, (Wd @ ld)[None, :] - PQd @ Wd)
+            W = W + W0 @ (phi * second)
+        kzi = np.where(kz == 0, 0, 1.0 / np.where(kz == 0, 1, kz))
+        H = Q @ W @ np.diag(kzi)
+        self.kz_norm[layer_num], self.E_eigvec[layer_num], self.H_eigvec[layer_num] = kz, W, H
+        # rcwa.py:646 reads V0iV = solve(Vf, H); Vf^-1 = -Vf for the free-space matrix
+        S11, S12 = self._smatrix_from_modes(W, H, kz, self.thickness[layer_num])
+        self.S11[layer_num] = S11
+        self.S12[layer_num] = S12
+        self.S21[layer_num] = S12
+        self.S22[layer_num] = S11
+
+    # ------------------------------------------------------- global S
+    def rs_prod(self, Sm, Sn):
+        # rcwa.py:695-722, stable-inverse branch
+        T1 = self.I2 - Sm[2] @ Sn[1]
+        T2 = self.I2 - Sn[1] @ Sm[2]
+        S11 = Sn[0] @ np.linalg.solve(T1, Sm[0])
+        S21 = Sm[1] + Sm[3] @ np.linalg.solve(T2, Sn[1] @ Sm[0])
+        S12 = Sn[2] + Sn[0] @ np.linalg.solve(T1, Sm[2] @ Sn[3])
+        S22 = Sm[3] @ np.linalg.solve(T2, Sn[3])
+        return [S11, S21, S12, S22]
+
+    def rs_power(self, S, n):
+        # rcwa.py:724-736
+        Z = np.zeros_like(self.I2)
+        res = [self.I2, Z, Z, self.I2]
+        while n:
+            if n & 1:
+                res = self.rs_prod(res, S)
+            S = self.rs_prod(S, S)
+            n >>= 1
+        return res
+
+    def layer(self, i):
+        return [self.S11[i], self.S21[i], self.S12[i], self.S22[i]]
+
+    def solve_global_smatrix(self):
+        # rcwa.py:159-192
+        Z = np.zeros_like(self.I2)
+        S = self.layer(0) if self.layer_N > 0 else [self.I2, Z, Z, self.I2]
+        for i in range(1, self.layer_N):
+            S = self.rs_prod(S, self.layer(i))
+        if self.has_in:
+            S = self.rs_prod(self.Sin, S)
+        if self.has_out:
+            S = self.rs_prod(S, self.Sout)
+        self.S = S
+        return S
+
+    def solve_global_smatrix_enhanced(self, start_index, end_index, repeat_number):
+        # rcwa.py:196-223
+        Z = np.zeros_like(self.I2)
+        S = [self.I2, Z, Z, self.I2]
+        for i in range(start_index):
+            S = self.rs_prod(S, self.layer(i))
+        G = [self.I2, Z, Z, self.I2]
+        for i in range(start_index, end_index):
+            G = self.rs_prod(G, self.layer(i))
+        S = self.rs_prod(S, self.rs_power(G, repeat_number))
+        for i in range(end_index, self.layer_N):
+            S = self.rs_prod(S, self.layer(i))
+        if self.has_in:
+            S = self.rs_prod(self.Sin, S)
+        if self.has_out:
+            S = self.rs_prod(S, self.Sout)
+        self.S = S
+        return S
+
+    # ---------------------------------------------------- source, fields
+    def matching_indices(self, orders):
+        # rcwa.py:340-347
+        o = np.array(orders, dtype=np.int64).reshape(-1, 2)
+        o[:, 0] = np.clip(o[:, 0], -self.order[0], self.order[0])
+        o[:, 1] = np.clip(o[:, 1], -self.order[1], self.order[1])
+        return len(self.order_y) * (o[:, 0] + self.order[0]) + o[:, 1] + self.order[1]
+
+    def source_fourier(self, amplitude, orders, direction="forward"):
+        # rcwa.py:237-267
+        amp = np.asarray(amplitude, dtype=np.complex128).reshape(-1, 2)
+        idx = self.matching_indices(orders)
+        self.source_direction = "backward" if direction in ("b", "backward") else "forward"
+        Ei = np.zeros((2 * self.order_N, 1), dtype=np.complex128)
+        Ei[idx, 0] = amp[:, 0]
+        Ei[idx + self.order_N, 0] = amp[:, 1]
+        self.E_i = Ei
+
+    def source_planewave(self, amplitude=(1.0, 0.0), direction="forward"):
+        self.source_fourier(amplitude, [0, 0], direction)
+
+    def _port(self):
+        if self.angle_layer == "input":
+            return 1 if self.source_direction == "forward" else 3
+        return 0 if self.source_direction == "forward" else 2
+
+    def order_fields(self, eps):
+        """Ex, Ey, Ez order amplitudes at the port of the source direction
+        (rcwa.py:285-304)."""
+        kz = _sqrt_upper(eps - self.Kx_dn ** 2 - self.Ky_dn ** 2)
+        if self.angle_layer != "input":
+            kz = -kz
+        Exy = self.S[self._port()] @ self.E_i
+        N = self.order_N
+        Ex, Ey = Exy[:N, 0], Exy[N:, 0]
+        Ez = (self.Kx_dn * Ex + self.Ky_dn * Ey) / kz
+        return Ex, Ey, Ez
+
+    def field_xy(self, x_axis, y_axis):
+        # rcwa.py:269-317
+        eps = self.eps_in if self.angle_layer == "input" else self.eps_out
+        Ex, Ey, Ez = self.order_fields(eps)
+        x = np.asarray(x_axis, dtype=np.float64).reshape(-1, 1, 1)
+        y = np.asarray(y_axis, dtype=np.float64).reshape(1, -1, 1)
+        ph = np.exp(1j * self.omega * (self.Kx_dn.reshape(1, 1, -1) * x + self.Ky_dn.reshape(1, 1, -1) * y))
+        return tuple(np.sum(c.reshape(1, 1, -1) * ph, axis=2) for c in (Ex, Ey, Ez))
+
+    def floquet_mode(self):
+        # rcwa.py:319-338: free-space kz with no sign change for either layer
+        Exy = self.S[self._port()] @ self.E_i
+        N = self.order_N
+        Ex, Ey = Exy[:N, 0], Exy[N:, 0]
+        kz = _sqrt_upper(1.0 - self.Kx_dn ** 2 - self.Ky_dn ** 2)
+        Ez = (self.Kx_dn * Ex + self.Ky_dn * Ey) / kz
+        shape = (len(self.order_x), len(self.order_y))
+        return Ex.reshape(shape), Ey.reshape(shape), Ez.reshape(shape)
+
+    # ------------------------------------------------------- diagnostics
+    def efficiencies(self):
+        """Reflected and transmitted power per order for the current source
+        (lossless input/output media): the Poynting flux of each order over
+        the incident flux."""
+        N = self.order_N
+        Ei = self.E_i[:, 0]
+
+        def flux(E, eps):
+            kz = _sqrt_upper(eps - self.Kx_dn ** 2 - self.Ky_dn ** 2)
+            ex, ey = E[:N], E[N:]
+            ez = (self.Kx_dn * ex + self.Ky_dn * ey) / kz
+            return np.real(kz) * (np.abs(ex) ** 2 + np.abs(ey) ** 2 + np.abs(ez) ** 2)
+
+        inc = flux(Ei, self.eps_in).sum()
+        R = flux(self.S[1] @ Ei, self.eps_in) / inc
+        T = flux(self.S[0] @ Ei, self.eps_out) / inc
+        return R, T

@@ -147,6 +147,25 @@ int oi_socs_image(oi_plan* plan, const void* field, int kind, int cin,
                   const int* chans, int batch, const void* kern, int nkern,
                   float* out, void* stream);
 
+/* No-allocation form of oi_socs_image.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  It takes
+ * the geometry inputs of oi_plan_create instead of a plan (max_batch and
+ * max_kernels are ignored: kernel groups are chosen as for a plan of
+ * `batch` images, so each image is summed as oi_socs_image sums it with such
+ * a plan) and calls no hipMalloc, hipFree or other allocating routine.
+ * Twiddle tables and scratch live in one static device array of
+ * OI_NOALLOC_SCRATCH_BYTES per GPU, held from module load until the process
+ * ends; the batch runs in passes of as many images as fit.  A geometry whose
+ * single image needs more than that fails with OI_ERR_INVALID.  Every call
+ * on one GPU shares the array: calls that overlap on one GPU (several
+ * streams or host threads) race and give undefined images.  Enqueued on
+ * `stream`; no synchronisation.                                              */
+#define OI_NOALLOC_SCRATCH_BYTES (32u << 20)
+int oi_socs_image_noalloc(const oi_geometry* geo, const void* field, int kind, int cin,
+                          const int* chans, int batch, const void* kern, int nkern,
+                          float* out, void* stream);
+
 /* Full centred spectrum Hk = scale * fftshift(ifft2(ifftshift(h))) of one
  * batch of fields, written channels-last c64 [batch][ny][nx][nchan].
  * Replaces Scatter.CalculateSpectrum's Hk (Scatter.py:20-23) with

@@ -68,6 +68,8 @@ SIGNATURES = {
     "oi_coherent_image": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int, c_void_p, c_void_p]),
     "oi_socs_image": (c_int, [c_void_p, c_void_p, c_int, c_int, P_int, c_int, c_void_p, c_int,
                               c_void_p, c_void_p]),
+    "oi_socs_image_noalloc": (c_int, [ctypes.POINTER(Geometry), c_void_p, c_int, c_int, P_int, c_int,
+                                      c_void_p, c_int, c_void_p, c_void_p]),
     "oi_spectrum_full": (c_int, [c_void_p, c_int, c_int, P_int, c_int, c_int, c_int, c_int, c_float,
                                  c_void_p, c_void_p, c_void_p]),
     "oi_kernel_support": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p]),

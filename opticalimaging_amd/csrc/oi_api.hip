@@ -2,6 +2,7 @@
 // points (spectrum window, coherent image, SOCS image, full spectrum).
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -692,4 +693,150 @@ extern "C" int oi_socs_adjoint(oi_plan* p, const float* mask, const float* grad_
   if (rc) return rc;
   return launch_adjoint(adj_geom(p), grad_image, p->spec, batch, static_cast<const float2*>(kern),
                         nkern, work, grad_mask, st);
+}
+
+// --------------------------------------------- no-allocation entry point
+// oi_socs_image_noalloc runs the same launches as oi_socs_image with the
+// plan's twiddle tables and scratch carved from `scratch`, a fixed device
+// array that every GPU holds from module load until the process ends.  Host
+// code never passes `scratch` itself: its device address on the current GPU
+// comes from hipGetSymbolAddress, taken once per device on the first call.
+// Calls on one GPU share the array, so calls that overlap on one GPU race.
+// No buffer in it needs to start at zero: every launch writes what it reads.
+namespace {
+constexpr size_t kNoAllocScratch = (size_t)OI_NOALLOC_SCRATCH_BYTES;
+constexpr int kMaxDevices = 64;
+constexpr size_t kAlign = 256;
+std::mutex g_scratch_mu;
+void* g_scratch_dev[kMaxDevices] = {};
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratch];
+
+// exp(-2 pi i m / n) in float64, rounded once (the host tables of
+// make_twiddles, computed on the device).
+__global__ void k_twiddle_table(float2* __restrict__ tw, int n) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m < n) {
+    const double ang = 2.0 * M_PI * (double)m / (double)n;
+    tw[m] = make_float2((float)cos(ang), (float)-sin(ang));
+  }
+}
+
+// Device address of `scratch` on the current GPU (looked up once per device).
+static int scratch_of_current_device(unsigned char** out) {
+  int dev = -1;
+  OI_HIP_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices)
+    return fail(OI_ERR_UNSUPPORTED, "socs_image_noalloc: device index beyond 63");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return fail(OI_ERR_RUNTIME, std::string("socs_image_noalloc: hipGetSymbolAddress failed: ") +
+                                      hipGetErrorString(e));
+    g_scratch_dev[dev] = ptr;
+  }
+  *out = static_cast<unsigned char*>(g_scratch_dev[dev]);
+  return OI_OK;
+}
+
+static size_t align_up(size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
+
+static size_t aligned_total(const ScratchLayout& s) {
+  return align_up(s.T) + align_up(s.spec) + align_up(s.R) + align_up(s.U) + align_up(s.Ic) +
+         align_up(s.Ihat) + align_up(s.part) + align_up(s.Bbuf);
+}
+
+extern "C" int oi_socs_image_noalloc(const oi_geometry* geo, const void* field, int kind, int cin,
+                                     const int* chans, int batch, const void* kern, int nkern,
+                                     float* out, void* stream) {
+  if (!geo || !field || !kern || !out) return fail(OI_ERR_INVALID, "socs_image_noalloc: null argument");
+  if (batch < 1) return fail(OI_ERR_INVALID, "socs_image_noalloc: batch must be >= 1");
+  if (nkern < 1) return fail(OI_ERR_INVALID, "socs_image_noalloc: nkern must be >= 1");
+  // Kernel groups are chosen as for a plan of `batch` images and `nkern`
+  // kernels, so each image is summed exactly as oi_socs_image sums it there.
+  oi_geometry in = *geo;
+  in.max_batch = batch;
+  in.max_kernels = nkern;
+  oi_geometry g;
+  int rc = plan_geometry(in, g);
+  if (rc) return rc;
+  // twiddle tables first, one per distinct length
+  const int sizes[5] = {g.nx, g.ny, g.lr, g.lc, g.rows};
+  FftDesc d[5];
+  size_t tw_off[5];
+  size_t used = 0;
+  for (int i = 0; i < 5; ++i) {
+    if (!fft_factor(sizes[i], d[i]))
+      return fail(OI_ERR_UNSUPPORTED, "socs_image_noalloc: FFT length " + std::to_string(sizes[i]) +
+                                          " is not a product of 2, 3 and 5");
+    tw_off[i] = used;
+    for (int j = 0; j < i; ++j)
+      if (sizes[j] == sizes[i]) tw_off[i] = tw_off[j];
+    if (tw_off[i] == used) used += align_up((size_t)sizes[i] * sizeof(float2));
+  }
+  const size_t per_image = aligned_total(scratch_layout(g, 1));
+  if (used + per_image > kNoAllocScratch)
+    return fail(OI_ERR_INVALID, "socs_image_noalloc: one image needs " +
+                                    std::to_string(used + per_image) + " bytes of scratch, beyond the " +
+                                    std::to_string(kNoAllocScratch) + " bytes this entry point holds");
+  // images per pass: align_up(c x) <= c align_up(x), so c images fit in c
+  // one-image layouts
+  const size_t fit = (kNoAllocScratch - used) / per_image;
+  const int chunk = fit < (size_t)batch ? (int)fit : batch;
+  unsigned char* base = nullptr;
+  rc = scratch_of_current_device(&base);
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  oi_plan pl;  // on the stack: pointers into `scratch`, nothing owned
+  pl.g = g;
+  pl.g.max_batch = chunk;
+  for (int i = 0; i < 5; ++i) {
+    d[i].tw = reinterpret_cast<const float2*>(base + tw_off[i]);
+    bool first = true;
+    for (int j = 0; j < i; ++j) first &= (sizes[j] != sizes[i]);
+    if (first) {
+      hipLaunchKernelGGL(k_twiddle_table, dim3((sizes[i] + 255) / 256), dim3(256), 0, st,
+                         reinterpret_cast<float2*>(base + tw_off[i]), sizes[i]);
+      OI_LAUNCH_CHECK("twiddle_table");
+    }
+  }
+  pl.fx = d[0];
+  pl.fy = d[1];
+  pl.flr = d[2];
+  pl.flc = d[3];
+  pl.frows = d[4];
+  const ScratchLayout sl = scratch_layout(pl.g, (size_t)chunk);
+  unsigned char* cur = base + used;
+  auto carve = [&cur](size_t bytes) -> unsigned char* {
+    unsigned char* p = bytes ? cur : nullptr;
+    cur += align_up(bytes);
+    return p;
+  };
+  pl.T = reinterpret_cast<float2*>(carve(sl.T));
+  pl.spec = reinterpret_cast<float2*>(carve(sl.spec));
+  pl.R = reinterpret_cast<float2*>(carve(sl.R));
+  pl.U = reinterpret_cast<float2*>(carve(sl.U));
+  pl.Ic = reinterpret_cast<float*>(carve(sl.Ic));
+  pl.Ihat = reinterpret_cast<float2*>(carve(sl.Ihat));
+  pl.part = reinterpret_cast<float*>(carve(sl.part));
+  pl.Bbuf = reinterpret_cast<float2*>(carve(sl.Bbuf));
+  if ((size_t)(cur - base) > kNoAllocScratch)
+    return fail(OI_ERR_RUNTIME, "socs_image_noalloc: scratch layout overflow");
+  const size_t pix = (size_t)g.ny * g.nx;
+  const size_t in_stride = kind == OI_FIELD_REAL ? pix * sizeof(float)
+                                                 : pix * (size_t)(cin > 0 ? cin : 1) * sizeof(float2);
+  for (int b0 = 0; b0 < batch; b0 += chunk) {
+    const int nb = batch - b0 < chunk ? batch - b0 : chunk;
+    const void* fb = static_cast<const unsigned char*>(field) + (size_t)b0 * in_stride;
+    rc = check_field(&pl, fb, kind, cin, chans, nb);
+    if (rc) return rc;
+    rc = spectrum_impl(&pl, fb, kind, cin, chans, nb, pl.spec, st);
+    if (rc) return rc;
+    rc = coherent_impl(&pl, pl.spec, nb, kern, nkern, out + (size_t)b0 * pix, nullptr, st);
+    if (rc) return rc;
+  }
+  return OI_OK;
 }

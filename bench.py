@@ -297,6 +297,61 @@ def abbe_configs_leg():
     return out
 
 
+def rcwa_leg():
+    """The near-field producer on the stack of the reference's demo pipeline
+    (ImagingPipeline.py:55-98: 365 nm, 3000 x 1500 nm, orders 5 x 5 = 121,
+    oxide 1e5 nm / TiN 21 nm / TEOS 100 nm patterned 200 x 100, SiN 20 nm and
+    substrate 5 nm homogeneous), its 256 x 128 near field, and the source-
+    weight gradient of the 128^2 scalar Abbe configuration (120 points)."""
+    import numpy as np_
+    from opticalimaging_amd import SMO, Scatter, workloads
+    from opticalimaging_amd.rcwa import rcwa
+    geometry = np_.zeros((200, 100))
+    geometry[:, 25:50] = 1
+    geometry[50:100, :] = 1
+    n_ox, n_tin, n_teos = 1.4745, 2.0669 + 1.0563j, 1.380
+    n_sin, n_sub = 2.1222, 6.5271 + 2.6672j
+    xs = (3000.0 / 256) * (torch.arange(256) + 0.5)
+    ys = (1500.0 / 128) * (torch.arange(128) + 0.5)
+
+    def run():
+        s = rcwa(freq=1 / 365.0, order=[5, 5], L=[3000.0, 1500.0], dtype=torch.complex128)
+        s.add_input_layer(eps=1.0)
+        s.set_incident_angle(inc_ang=0, azi_ang=0)
+        for t_, n_ in zip([1e5, 21, 100], [n_ox, n_tin, n_teos]):
+            s.add_layer(thickness=t_, eps=torch.tensor(geometry * (n_ ** 2 - 1) + 1.0))
+        s.add_layer(thickness=20, eps=n_sin ** 2)
+        s.add_layer(thickness=5, eps=n_sub ** 2)
+        s.solve_global_smatrix()
+        s.source_planewave(amplitude=[0.0, 1.0], direction="forward")
+        f = s.field_xy(xs, ys)
+        torch.cuda.synchronize()
+        return f
+    run()
+    times = []
+    for _ in range(3):
+        t = time.perf_counter()
+        run()
+        times.append(time.perf_counter() - t)
+    out = {"stack": "ImagingPipeline demo: 5 layers, 121 orders (2N = 242), near field 256 x 128",
+           "ms_per_solve_and_field": round(float(np_.median(times)) * 1e3, 2)}
+    mask, src, proj, num, period = workloads.scalar_abbe_128()
+    sc = Scatter()
+    sc.Period_X = sc.Period_Y = period
+    sc.ScatterField = workloads.field_from_mask(mask).cuda()
+    prob = SMO.abbe_source_problem(sc.ScatterField, src, sc, proj, num)
+    g = torch.ones((1, 128, 128), device="cuda")
+    prob.source_gradient(prob.V0, g)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(5):
+        prob.source_gradient(prob.V0, g)
+    torch.cuda.synchronize()
+    out["source_gradient_128_ms"] = round((time.perf_counter() - t) / 5 * 1e3, 3)
+    out["source_points"] = prob.nsrc
+    return out
+
+
 def ilt_leg(steps):
     """||I(sigmoid(theta)) - T||^2 descent at 1024^2 x 16 with the adjoint kernel."""
     from opticalimaging_amd import ILT, workloads
@@ -639,6 +694,10 @@ def main():
         if args.ilt_steps > 0:
             line["ilt"] = ilt_leg(args.ilt_steps)
             log(f"ilt: {json.dumps(line['ilt'])}")
+
+        # ---------------- RCWA near field of the demo stack; source gradient
+        line["rcwa_and_source_gradient"] = rcwa_leg()
+        log(f"rcwa / source gradient: {json.dumps(line['rcwa_and_source_gradient'])}")
 
         # ---------------- CPU baseline: the float64 oracle, timed on a bounded
         # sample; its outputs are also the reference of the accuracy fields.

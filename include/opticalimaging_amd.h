@@ -324,6 +324,41 @@ int oi_ilt_residual(const float* image, const float* target, float* resid2, floa
 int oi_ilt_update(float* theta, const float* grad_m, const float* m, float lr, int batch,
                   size_t npix, float* work, float* m_next, void* stream);
 
+/* ----------------------------------------------------------------- RCWA --
+ * Stages of the RCWA near-field producer (rcwa.py:4-872) that are not dense
+ * linear algebra; the products, solves and inverses between them run in
+ * rocBLAS/rocSOLVER (opticalimaging_amd/rcwa.py).  Fourier orders
+ * (ox, oy), |ox| <= Ox, |oy| <= Oy, are flattened as
+ * i = (ox + Ox)(2 Oy + 1) + (oy + Oy) (rcwa.py:340-347, 356-361);
+ * N = (2 Ox + 1)(2 Oy + 1).  Arrays are c128 (interleaved float64 pairs),
+ * row-major; kx, ky are the N normalised wave-vector components (float64).
+ *
+ * oi_rcwa_convmat: convolution matrix of a permittivity grid,
+ *   E[i][j] = F[(ox_i - ox_j) mod n0][(oy_i - oy_j) mod n1], F = fft2(eps) / (n0 n1)
+ *   already computed (c128 [n0][n1]); E : c128 [N][N].  Replaces the gather
+ *   of rcwa.py:435-450.                                                      */
+int oi_rcwa_convmat(const void* F, int n0, int n1, int ox, int oy, void* E, void* stream);
+
+/* oi_rcwa_pq: P and Q of a layer (rcwa.py:470-497) from E (c128 [N][N]) and
+ *   EiK = [E^-1 Kx | E^-1 Ky] (c128 [N][2N]); P, Q : c128 [2N][2N].           */
+int oi_rcwa_pq(const void* E, const void* EiK, const double* kx, const double* ky, int N,
+               void* P, void* Q, void* stream);
+
+/* oi_rcwa_modes: mode matrix V (c128 [2N][2N]) and kz (c128 [N], may be
+ *   null) of a homogeneous medium of permittivity eps (get_V,
+ *   rcwa.py:865-872; kz on the branch with Im >= 0).                         */
+int oi_rcwa_modes(double eps_re, double eps_im, const double* kx, const double* ky, int N,
+                  void* V, void* kz, void* stream);
+
+/* oi_rcwa_fields: field synthesis on an x-y grid (rcwa.py:306-315),
+ *   out[x][y][c] = sum_o amp[c][o] exp(i omega (kxo[o / noy] xs[x] + kyo[o % noy] ys[y])),
+ *   amp : c128 [3][nox noy] (Ex, Ey, Ez order amplitudes), kxo : float64
+ *   [nox], kyo : float64 [noy], xs : float64 [nx], ys : float64 [ny];
+ *   out : c128 [nx][ny][3] (the ScatterField layout).  nox + noy <= 256.     */
+int oi_rcwa_fields(const void* amp, int nox, int noy, const double* kxo, const double* kyo,
+                   double omega, const double* xs, int nx, const double* ys, int ny, void* out,
+                   void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -56,6 +56,7 @@ class PolarizationParameters:
 def _grid(cx, cy):
     s = SourceData(len(cx), len(cy))
     s.X, s.Y = torch.meshgrid(cx, cy, indexing="ij")
+    s.Value = torch.zeros((len(cx), len(cy)), device=cx.device)
     return s
 
 
@@ -153,10 +154,11 @@ class Source:
         self.PolarizationVector = [1.0, 0.0]
         self.PolarizationType = None
 
-    def Calc_SourceSimple(self):
+    def Calc_SourceSimple(self, device=None):
         """Lit points only: weights below 1e-5 are dropped (Source.py:47-54).
-        Sets and returns ``self.source_data`` with 1-D X, Y, Value."""
-        self.source_data = self.Calc_SourceAll()
+        Sets and returns ``self.source_data`` with 1-D X, Y, Value (on
+        `device` when given, see Calc_SourceAll)."""
+        self.source_data = self.Calc_SourceAll(device=device)
         low = self.source_data.Value < 1e-5
         self.source_data.X = self.source_data.X[~low]
         self.source_data.Y = self.source_data.Y[~low]
@@ -169,10 +171,19 @@ class Source:
         self.source_data.Value = torch.where((data.X.pow(2) + data.Y.pow(2)) > 1, 0, data.Value)
         return self.source_data
 
-    def Calc_SourceAll(self):
+    def Calc_SourceAll(self, device=None):
         """P x P source: shape -> Source_Mask -> optional PSF blur -> centre
-        pixel zeroed -> flattened (Source.py:64-180)."""
+        pixel zeroed -> flattened (Source.py:64-180).
+
+        With `device` (e.g. the HIP device) the shape rules, mask, blur and
+        flattening run there on the float32 coordinate grid built on the host
+        (torch.linspace, as the reference builds it); the elementwise float32
+        arithmetic and comparisons are the same operations, so the lit set and
+        weights equal the host's bit for bit except under the PSF blur, whose
+        device convolution sums in another order (float32 rounding)."""
         coord = torch.linspace(-1, 1, self.PntNum)
+        if device is not None:
+            coord = coord.to(device)
         shape = self.Shape.lower()
         if shape == "pixel":
             n2 = self.PntNum ** 2
@@ -182,9 +193,9 @@ class Source:
                 raise ValueError("Source Matrix Size Y and difined PntNum are not matched ")
             if self.SPointValue.numel() != n2:
                 raise ValueError("Source Matrix Size Value and difined PntNum are not matched ")
-            self.source_data.X = self.SPointX
-            self.source_data.Y = self.SPointY
-            self.source_data.Value = self.SPointValue
+            self.source_data.X = self.SPointX.to(coord.device)
+            self.source_data.Y = self.SPointY.to(coord.device)
+            self.source_data.Value = self.SPointValue.to(coord.device)
         elif shape == "annular":
             self.source_data = CalculateAnnularSourceMatrix(self.SigmaOut, self.SigmaIn, coord, coord)
         elif shape == "quasar":
@@ -210,10 +221,12 @@ class Source:
 
         sx, sy = self.source_data.Value.size()
         if self.Source_Mask is not None:
-            self.source_data.Value = torch.mul(self.source_data.Value, self.Source_Mask)
+            self.source_data.Value = torch.mul(self.source_data.Value,
+                                               torch.as_tensor(self.Source_Mask).to(coord.device))
         if sx == sy:
             if self.PSFEnable:
-                self.source_data.Value = conv2(self.source_data.Value, self._psf_kernel(), mode="same")
+                self.source_data.Value = conv2(self.source_data.Value,
+                                               self._psf_kernel().to(coord.device), mode="same")
             self.source_data.Value[int((sx - 1) / 2), int((sy - 1) / 2)] = 0
             self.source_data = ConvertSourceMatrix2SourceData(self.source_data)
         return self.source_data

@@ -100,6 +100,13 @@ SIGNATURES = {
                                  c_void_p, c_void_p]),
     "oi_ilt_residual": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_size_t, c_void_p,
                                 c_void_p]),
+    "oi_rcwa_convmat": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_void_p]),
+    "oi_rcwa_pq": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p,
+                           c_void_p]),
+    "oi_rcwa_modes": (c_int, [c_double, c_double, c_void_p, c_void_p, c_int, c_void_p, c_void_p,
+                              c_void_p]),
+    "oi_rcwa_fields": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_double, c_void_p,
+                               c_int, c_void_p, c_int, c_void_p, c_void_p]),
     "oi_ilt_update": (c_int, [c_void_p, c_void_p, c_void_p, c_float, c_int, c_size_t, c_void_p,
                               c_void_p, c_void_p]),
 }

@@ -23,7 +23,7 @@ LIB_PATH = os.path.join(PKG, LIB_NAME)
 ARCH = os.environ.get("OI_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["oi_api.hip", "oi_imaging.hip", "oi_fastpath.hip", "oi_pupil.hip", "oi_tcc.hip",
-           "oi_eigh.hip", "oi_adjoint.hip", "oi_ilt.hip"]
+           "oi_eigh.hip", "oi_adjoint.hip", "oi_ilt.hip", "oi_rcwa.hip"]
 
 COMMON_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",

@@ -177,11 +177,21 @@ __device__ __forceinline__ void tcc_panel_store(unsigned short* lds, const u32x4
   }
 }
 
+// Output tile staging: the workgroup's 64 x 64 result goes through LDS so
+// that each store instruction writes one 64-entry row segment (512 B) of the
+// TCC, and the mirrored tile one row of the conjugate transpose, instead of
+// four 128-byte pieces of four rows.  It shares the LDS of the operand panels.
+constexpr int kOutStride = kTileB + 1;
+constexpr size_t kPanBytes = (size_t)kPanels * kTileB * kPanelRow * sizeof(unsigned short);
+constexpr size_t kOutBytes = (size_t)kTileB * kOutStride * sizeof(float2);
+constexpr size_t kTccLds = kPanBytes > kOutBytes ? kPanBytes : kOutBytes;
+
 __global__ __launch_bounds__(256) void k_tcc_bf16(const unsigned short* __restrict__ planes, int M,
                                                   int Kp, int nt, double alpha,
                                                   float2* __restrict__ tcc) {
   constexpr int FT = kTileB / 32;               // MFMA tiles per wave and dimension
-  __shared__ unsigned short pan[kPanels * kTileB * kPanelRow];
+  __shared__ __align__(16) unsigned char lds_raw[kTccLds];
+  unsigned short* pan = reinterpret_cast<unsigned short*>(lds_raw);
   const size_t plane = (size_t)M * Kp;
   int I, J;
   tile_pair(blockIdx.x, nt, I, J);
@@ -227,14 +237,34 @@ __global__ __launch_bounds__(256) void k_tcc_bf16(const unsigned short* __restri
         accI[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], ih[b], accI[a][b], 0, 0, 0);
       }
   }
-  const bool mirror = I != J;
-  __shared__ float2 mbuf_all[4 * 16 * kMirStride];
-  float2* mbuf = mbuf_all + (threadIdx.x >> 6) * 16 * kMirStride;
+  // stage the tile (alpha applied, rounded to float32) in LDS
+  float2* otile = reinterpret_cast<float2*>(lds_raw);
+  __syncthreads();                              // the last step's panel reads are done
 #pragma unroll
   for (int a = 0; a < FT; ++a)
 #pragma unroll
     for (int b = 0; b < FT; ++b)
-      store_frag<false>(tcc, M, r0 + a * 16, c0 + b * 16, accR[a][b], accI[a][b], alpha, mirror, mbuf);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int rl = ar + a * 16 + (lane >> 4) * 4 + i, cl = bc + b * 16 + (lane & 15);
+        otile[rl * kOutStride + cl] = make_float2((float)(alpha * (double)accR[a][b][i]),
+                                                  (float)(alpha * (double)accI[a][b][i]));
+      }
+  __syncthreads();
+  // tile (I, J): wave w writes rows w, w + 4, ...; lane = column
+  const int col = J * kTileB + lane;
+  for (int rl = wave; rl < kTileB; rl += 4) {
+    const int row = I * kTileB + rl;
+    if (row < M && col < M) tcc[(size_t)row * M + col] = otile[rl * kOutStride + lane];
+  }
+  if (I == J) return;                           // uniform over the workgroup
+  // tile (J, I) = conjugate transpose: row J*64 + cl holds column cl of the tile
+  const int mcol = I * kTileB + lane;
+  for (int cl = wave; cl < kTileB; cl += 4) {
+    const int row = J * kTileB + cl;
+    const float2 v = otile[lane * kOutStride + cl];
+    if (row < M && mcol < M) tcc[(size_t)row * M + mcol] = make_float2(v.x, -v.y);
+  }
 }
 
 // fp32 MFMA 16x16x4: lane fragment is one value at k = l>>4; a 16-wide K step

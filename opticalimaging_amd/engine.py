@@ -342,13 +342,20 @@ def field_encoding(field):
     if f.dim() == 3:
         f = f.unsqueeze(0)
     cin = f.shape[-1]
-    live = [c for c in range(cin) if bool((f[..., c] != 0).any())]
+    flat = f.reshape(-1, cin)
+    # which channels are non-zero, and which carry an imaginary part: one
+    # device-to-host transfer for both
+    if torch.is_complex(f):
+        stats = torch.stack([(flat != 0).any(0), (flat.imag != 0).any(0)]).tolist()
+    else:
+        stats = [(flat != 0).any(0).tolist(), [False] * cin]
+    live = [c for c in range(cin) if stats[0][c]]
     if not live:
         live = [0]
-    if len(live) == 1 and not torch.is_complex(f):
-        return (f[..., live[0]].to(torch.float32).contiguous(), nat.OI_FIELD_REAL, 1, [0])
-    if len(live) == 1 and torch.is_complex(f) and bool((f[..., live[0]].imag == 0).all()):
-        return (f[..., live[0]].real.to(torch.float32).contiguous(), nat.OI_FIELD_REAL, 1, [0])
+    if len(live) == 1 and not stats[1][live[0]]:
+        h = f[..., live[0]]
+        h = h.real if torch.is_complex(h) else h
+        return (h.to(torch.float32).contiguous(), nat.OI_FIELD_REAL, 1, [0])
     return f.to(torch.complex64).contiguous(), nat.OI_FIELD_C64_CHLAST, cin, live
 
 

@@ -40,7 +40,7 @@ __all__ = [
     "shifted_pupil_matrix", "get_tcc_matrix", "tcc_matrix", "eig_desc",
     "socs_count", "decompose_tcc_socs", "decompose_lowrank", "embed_kernels",
     "socs_image", "hopkins_image", "dft2_bruteforce", "clear_field_intensity",
-    "calculate_socs", "optimized_image",
+    "calculate_socs", "optimized_image", "abbe_source_terms",
 ]
 
 
@@ -316,17 +316,10 @@ def spectrum(field, period_x, period_y, NA, wavelength):
 # Abbe (CalculateAerialImage.py:18-103)
 # ---------------------------------------------------------------------------
 
-def abbe_image(src, scatter, proj, numerics):
-    """Partially coherent Abbe image, float64.  Returns
-    (Intensity (nFocus, Ny, Nx), ImageX, ImageY, ImageZ, PupilImage).
-
-    Validity of a pupil sample is decided as rho^2 <= 1 in float64
-    (the reference decides it in float32 at CalculateAerialImage.py:55-57)."""
+def _abbe_setup(src, scatter, proj, numerics):
     mode = _mode(numerics)
     h = _field(scatter)
-    ny, nx, nch = h.shape
     X, Y, V = source_points(src)
-    W = V.sum()
     lam = float(src.Wavelength)
     NA, n = float(proj.NA), float(proj.IndexImage)
     hk, f, g, pupil_img = spectrum(h, scatter.Period_X, scatter.Period_Y, NA, lam)
@@ -334,39 +327,77 @@ def abbe_image(src, scatter, proj, numerics):
     dfmdg = (f[1] - f[0]) * (g[1] - g[0])                 # :38
     rs, ts = np.hypot(X, Y), np.arctan2(Y, X)
     PX, PY = polarization(src, ts, rs)
-    ncomp = 3 if mode == "vector" else 1
-    live = [c for c in range(nch) if np.any(h[:, :, c] != 0)]   # zero channels add 0
+    live = [c for c in range(h.shape[2]) if np.any(h[:, :, c] != 0)]   # zero channels add 0
+    return dict(mode=mode, h=h, X=X, Y=Y, V=V, lam=lam, NA=NA, n=n, hk=hk, fm=fm, gm=gm,
+                dfmdg=dfmdg, PX=PX, PY=PY, live=live, pupil_img=pupil_img)
+
+
+def _abbe_source_term(c, j, z):
+    """sum over field channels and polarisation components of |fft2(S)|^2 for
+    source point j at focus z, unweighted and unshifted
+    (CalculateAerialImage.py:55-87)."""
+    ny, nx = c["h"].shape[:2]
+    fc = c["fm"] + c["X"][j]
+    gc = c["gm"] + c["Y"][j]
+    rho2 = fc ** 2 + gc ** 2
+    valid = rho2 <= 1                                      # :55-57
+    fv, gv = fc[valid], gc[valid]
+    r2 = rho2[valid]
+    th = np.arctan2(gv, fv)
+    pv = _pupil(r2, th, c["proj"], c["lam"], z)
+    if c["mode"] == "vector":                              # :68-74
+        mxx, myx, mxy, myy, mxz, myz = characteristic_matrix(fv, gv, c["NA"], c["n"])
+        PX, PY = c["PX"][j], c["PY"][j]
+        obl = [PX * mxx + PY * myx, PX * mxy + PY * myy, PX * mxz + PY * myz]
+    else:
+        obl = [1.0]
+    accj = np.zeros((ny, nx))
+    for ch in c["live"]:                                   # :76-87
+        base = c["hk"][:, :, ch][valid] * pv
+        for o in obl:
+            S = np.zeros((ny, nx), dtype=np.complex128)
+            S[valid] = base * o
+            accj += np.abs(np.fft.fft2(S)) ** 2
+    return accj
+
+
+def abbe_image(src, scatter, proj, numerics):
+    """Partially coherent Abbe image, float64.  Returns
+    (Intensity (nFocus, Ny, Nx), ImageX, ImageY, ImageZ, PupilImage).
+
+    Validity of a pupil sample is decided as rho^2 <= 1 in float64
+    (the reference decides it in float32 at CalculateAerialImage.py:55-57)."""
+    c = _abbe_setup(src, scatter, proj, numerics)
+    c["proj"] = proj
+    ny, nx = c["h"].shape[:2]
+    V = c["V"]
+    W = V.sum()
     foci = _foci(proj)
     out = np.zeros((len(foci), ny, nx))
     for fi, z in enumerate(foci):
         acc = np.zeros((ny, nx))
         for j in range(len(V)):
-            fc = fm + X[j]
-            gc = gm + Y[j]
-            rho2 = fc ** 2 + gc ** 2
-            valid = rho2 <= 1                              # :55-57
-            fv, gv = fc[valid], gc[valid]
-            r2 = rho2[valid]
-            th = np.arctan2(gv, fv)
-            pv = _pupil(r2, th, proj, lam, z)
-            if mode == "vector":                           # :68-74
-                mxx, myx, mxy, myy, mxz, myz = characteristic_matrix(fv, gv, NA, n)
-                obl = [PX[j] * mxx + PY[j] * myx, PX[j] * mxy + PY[j] * myy,
-                       PX[j] * mxz + PY[j] * myz]
-            else:
-                obl = [1.0]
-            accj = np.zeros((ny, nx))
-            for c in live:                                 # :76-87
-                base = hk[:, :, c][valid] * pv
-                for e in range(ncomp):
-                    S = np.zeros((ny, nx), dtype=np.complex128)
-                    S[valid] = base * obl[e]
-                    accj += np.abs(np.fft.fft2(S)) ** 2
-            acc += V[j] * accj                             # :89
-        out[fi] = n / W * dfmdg ** 2 * np.fft.fftshift(acc)   # :90-92
+            acc += V[j] * _abbe_source_term(c, j, z)       # :89
+        out[fi] = c["n"] / W * c["dfmdg"] ** 2 * np.fft.fftshift(acc)   # :90-92
     image_x = np.linspace(-scatter.Period_X / 2, scatter.Period_X / 2, nx)
     image_y = np.linspace(-scatter.Period_Y / 2, scatter.Period_Y / 2, ny)
-    return out, image_x, image_y, np.array(foci), pupil_img
+    return out, image_x, image_y, np.array(foci), c["pupil_img"]
+
+
+def abbe_source_terms(src, scatter, proj, numerics):
+    """Per-source unweighted intensities J (nFocus, Nsrc, Ny, Nx) with the
+    Abbe image I = (n / sum V) sum_s V_s J_s (CalculateAerialImage.py:46-92),
+    and (V, n): the image as an explicit function of the source weights, for
+    source-weight gradients.  Small grids only (Nsrc images per focus)."""
+    c = _abbe_setup(src, scatter, proj, numerics)
+    c["proj"] = proj
+    ny, nx = c["h"].shape[:2]
+    foci = _foci(proj)
+    J = np.zeros((len(foci), len(c["V"]), ny, nx))
+    for fi, z in enumerate(foci):
+        for j in range(len(c["V"])):
+            J[fi, j] = c["dfmdg"] ** 2 * np.fft.fftshift(_abbe_source_term(c, j, z))
+    return J, c["V"], c["n"]
 
 
 def clear_field_intensity(src, proj, numerics=None):

@@ -19,9 +19,12 @@ restated through scipy.linalg.expm (a Pade scaling-and-squaring method)
 instead of the reference's degree-18 polynomial, then converted to the
 layer S-matrix with the same transfer-to-scattering formulas.
 
-Deliberate difference: the output layer's mode matrix is get_V(eps_out)
-(the evident intent of rcwa.py:407-431, which assigns self.Vi and then reads
-an undefined self.Vo).
+Deliberate differences in the output coupling (rcwa.py:407-431), which the
+reference cannot reach (it assigns self.Vi and then reads an undefined
+self.Vo): the output medium's mode matrix is get_V(eps_out), and S21 = -S12
+as in the input coupling (the reference writes S21 = S12; with that sign a
+lossless stack does not conserve energy and a bare interface reflects with
+the wrong sign).
 
 Pins (tests/test_oracle_rcwa.py): energy conservation R + T = 1 for lossless
 gratings; the Airy closed form of a homogeneous slab at normal and oblique
@@ -105,10 +108,12 @@ class RcwaOracle:
             S12 = np.linalg.solve(a, b)
             self.Sin = [S11, -S12, S12, 2 * np.linalg.solve(a, self.Vf)]
         if self.has_out:
+            # the input coupling mirrored: S21 = -S12 (rcwa.py:418 writes S21 = S12,
+            # which breaks energy conservation and the Fresnel reflection sign)
             Vo, _ = self.get_V(self.eps_out)
             a, b = self.Vf + Vo, self.Vf - Vo
             S12 = -np.linalg.solve(a, b)
-            self.Sout = [2 * np.linalg.solve(a, self.Vf), S12, S12, 2 * np.linalg.solve(a, Vo)]
+            self.Sout = [2 * np.linalg.solve(a, self.Vf), -S12, S12, 2 * np.linalg.solve(a, Vo)]
         self.I2 = np.eye(n2, dtype=np.complex128)
 
     def get_V(self, eps):

@@ -245,6 +245,14 @@ def test_abbe_clear_field_closed_form():
     I, *_ = O.abbe_image(src, scatter(f, 1000.0, 1000.0), proj, numerics())
     assert np.allclose(I, 1.21476847, atol=1e-7)
     assert abs(O.clear_field_intensity(src, proj) - 1.21476847) < 1e-7
+    # per source point: a clear mask passes only the DC order, so each
+    # unweighted term is the constant squared obliquity at rho = |s|
+    J, V, n = O.abbe_source_terms(src, scatter(f, 1000.0, 1000.0), proj, numerics())
+    X, Y, _ = O.source_points(src)
+    r2 = X ** 2 + Y ** 2
+    ob2 = np.sqrt((1 - (0.85 / 100) ** 2 * r2) / (1 - 0.85 ** 2 * r2))
+    assert np.allclose(J[0], ob2[:, None, None], atol=1e-12)
+    assert np.allclose(n / V.sum() * np.einsum("s,sij->ij", V, J[0]), I[0], atol=1e-12)
 
 
 def test_abbe_dark_and_shift_equivariance():

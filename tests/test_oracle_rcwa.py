@@ -92,6 +92,29 @@ def test_lossless_grating_conserves_energy(orders, inc, azi, amp):
     assert R.sum() > 1e-3 and T.sum() > 0.3
 
 
+def test_output_coupling_fresnel_and_energy():
+    """A bare interface into an output half-space has the Fresnel amplitudes
+    (r = (n1 - n2)/(n1 + n2) for E, t = 2 n1/(n1 + n2)), and a lossless
+    grating between two dielectric half-spaces conserves energy."""
+    s = RcwaOracle(1 / 193.0, [0, 0], [600.0, 400.0])
+    s.add_input_layer(1.0)
+    s.add_output_layer(2.25)
+    s.set_incident_angle(0.0, 0.0)
+    S = s.solve_global_smatrix()
+    assert abs(S[0][0, 0] - 0.8) < 1e-12 and abs(S[1][0, 0] + 0.2) < 1e-12
+    geo = GOLD["geo"]
+    g = RcwaOracle(1 / 193.0, [4, 2], [600.0, 400.0])
+    g.add_input_layer(1.21)
+    g.add_output_layer(2.1)
+    g.set_incident_angle(math.radians(17), math.radians(40))
+    g.add_layer(50.0, geo * 1.25 + 1.0)
+    g.add_layer(35.0, np.roll(geo, 7, axis=1) * 2.1 + 1.0)
+    g.solve_global_smatrix()
+    g.source_planewave((0.3, 0.9j))
+    R, T = g.efficiencies()
+    assert abs(R.sum() + T.sum() - 1) < 1e-9
+
+
 def test_eigen_and_exponential_routes_agree():
     a = _grating_case([3, 2], 12.0, 25.0, fast=False)
     b = _grating_case([3, 2], 12.0, 25.0, fast=True)

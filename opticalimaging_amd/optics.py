@@ -53,6 +53,9 @@ def source_set(source) -> SourceSet:
     """Calc_SourceSimple (Source.py:47-54) plus polarisation, moved to the
     device.  Leaves ``source.source_data`` populated as the reference does."""
     dev = nat.device()
+    # host-side selection: the drop-in leaves source.source_data on the host
+    # as the reference does (Source.Calc_SourceSimple(device=...) gives the
+    # same lit set on the device for callers that keep it there)
     sd = source.Calc_SourceSimple()
     X = sd.X.reshape(-1).to(torch.float64)
     Y = sd.Y.reshape(-1).to(torch.float64)

@@ -129,17 +129,20 @@ class rcwa:
         self.Kx_norm_dn = self._kx.to(self._dtype)
         self.Ky_norm_dn = self._ky.to(self._dtype)
         self.Vf, _ = self._modes(1.0)
+        n2 = 2 * self.order_N
         if hasattr(self, 'Sin'):
             Vi, _ = self._modes(self.eps_in)
-            a, b = self.Vf + Vi, self.Vf - Vi
-            S12 = torch.linalg.solve(a, b)
-            self.Sin = [2 * torch.linalg.solve(a, Vi), -S12, S12, 2 * torch.linalg.solve(a, self.Vf)]
+            a = self.Vf + Vi
+            X = torch.linalg.solve(a, torch.cat([self.Vf - Vi, Vi, self.Vf], 1))
+            S12 = X[:, :n2]
+            self.Sin = [2 * X[:, n2:2 * n2], -S12, S12, 2 * X[:, 2 * n2:]]
         if hasattr(self, 'Sout'):
             # the input coupling mirrored, S21 = -S12 (see the module docstring)
             Vo, _ = self._modes(self.eps_out)
-            a, b = self.Vf + Vo, self.Vf - Vo
-            S12 = -torch.linalg.solve(a, b)
-            self.Sout = [2 * torch.linalg.solve(a, self.Vf), -S12, S12, 2 * torch.linalg.solve(a, Vo)]
+            a = self.Vf + Vo
+            X = torch.linalg.solve(a, torch.cat([self.Vf - Vo, self.Vf, Vo], 1))
+            S12 = -X[:, :n2]
+            self.Sout = [2 * X[:, n2:2 * n2], -S12, S12, 2 * X[:, 2 * n2:]]
 
     def _modes(self, eps):
         """(V, kz) of a homogeneous medium (get_V, rcwa.py:865-872)."""
@@ -260,21 +263,23 @@ class rcwa:
         T21 = 0.5 * (Ev - F)
         S11 = torch.linalg.inv(T22)
         S12 = -S11 @ T21
+        n2 = I.shape[0]
         for _ in range(nsplit):   # star product of two equal symmetric slices (rcwa.py:832-836)
-            R = I - S12 @ S12
-            S12, S11 = S12 + S11 @ torch.linalg.solve(R, S12 @ S11), S11 @ torch.linalg.solve(R, S11)
+            X = torch.linalg.solve(I - S12 @ S12, torch.cat([S12 @ S11, S11], 1))
+            S12, S11 = S12 + S11 @ X[:, :n2], S11 @ X[:, n2:]
         return S11, S12
 
     # ------------------------------------------------------- global S
     def _RS_prod(self, Sm, Sn):
         """Redheffer star product (rcwa.py:695-722)."""
         I = self._I2
-        T1 = I - Sm[2] @ Sn[1]
-        T2 = I - Sn[1] @ Sm[2]
-        S11 = Sn[0] @ torch.linalg.solve(T1, Sm[0])
-        S21 = Sm[1] + Sm[3] @ torch.linalg.solve(T2, Sn[1] @ Sm[0])
-        S12 = Sn[2] + Sn[0] @ torch.linalg.solve(T1, Sm[2] @ Sn[3])
-        S22 = Sm[3] @ torch.linalg.solve(T2, Sn[3])
+        n2 = I.shape[0]
+        X1 = torch.linalg.solve(I - Sm[2] @ Sn[1], torch.cat([Sm[0], Sm[2] @ Sn[3]], 1))
+        X2 = torch.linalg.solve(I - Sn[1] @ Sm[2], torch.cat([Sn[1] @ Sm[0], Sn[3]], 1))
+        S11 = Sn[0] @ X1[:, :n2]
+        S21 = Sm[1] + Sm[3] @ X2[:, :n2]
+        S12 = Sn[2] + Sn[0] @ X1[:, n2:]
+        S22 = Sm[3] @ X2[:, n2:]
         return S11, S21, S12, S22
 
     def _RS_power(self, S, n):

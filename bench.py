@@ -78,6 +78,8 @@ def parse():
     ap.add_argument("--ilt-steps", type=int, default=50, help="gradient steps of the ILT leg (0 skips)")
     ap.add_argument("--ilt-only", action="store_true",
                     help="only run the ILT leg and print its JSON (profiling runs)")
+    ap.add_argument("--abbe-only", action="store_true",
+                    help="only run the two Abbe configurations and print their JSON (profiling runs)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 device copies of MIB MiB (calibrates HBM byte counters)")
     return ap.parse_args()
@@ -434,6 +436,9 @@ def main():
         return
     if args.ilt_only:
         print(json.dumps({"ilt": ilt_leg(max(1, args.ilt_steps))}), flush=True)
+        return
+    if args.abbe_only:
+        print(json.dumps({"abbe_configs": abbe_configs_leg()}), flush=True)
         return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:

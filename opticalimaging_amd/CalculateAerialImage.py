@@ -42,8 +42,9 @@ def cartesian_to_polar(x, y):
     return torch.sqrt(x ** 2 + y ** 2), torch.atan2(y, x)
 
 
-def abbe_images(field, source, scatter, projector, numerics) -> torch.Tensor:
-    """Abbe intensities (nFocus, Ny, Nx) float32 on the device."""
+def abbe_images(field, source, scatter, projector, numerics, encoding=None) -> torch.Tensor:
+    """Abbe intensities (nFocus, Ny, Nx) float32 on the device.  ``encoding``
+    is engine.field_encoding(field) when the caller already has it."""
     vector = optics.imaging_mode(numerics)
     dev = nat.device()
     h = torch.as_tensor(field)
@@ -51,7 +52,7 @@ def abbe_images(field, source, scatter, projector, numerics) -> torch.Tensor:
     px, py = _pitches(scatter, projector, source)
     src = optics.source_set(source)
     k0r, wr, k0c, wc, br, bc = optics.abbe_window(src, py, px, ny, nx)
-    tens, kind, cin, live = engine.field_encoding(h)
+    tens, kind, cin, live = encoding if encoding is not None else engine.field_encoding(h)
     ncomp = 3 if vector else 1
     plan = engine.PLANS.get(ny=ny, nx=nx, k0r=k0r, wr=wr, k0c=k0c, wc=wc, nchan=len(live),
                             max_batch=1, band_r=br, band_c=bc, max_kernels=ncomp * src.n)
@@ -85,9 +86,10 @@ def CalculateAbbeImage(source, scatter, projector, numerics):
     on the ScatterField's device (CalculateAerialImage.py:18-103)."""
     h = torch.as_tensor(scatter.ScatterField)
     ny, nx = int(h.shape[0]), int(h.shape[1])
-    intensity = abbe_images(h, source, scatter, projector, numerics)
+    enc = engine.field_encoding(h)          # one device-to-host transfer per call
+    intensity = abbe_images(h, source, scatter, projector, numerics, encoding=enc)
     px, py = _pitches(scatter, projector, source)
-    pupil = pupil_image(h, px, py)
+    pupil = pupil_image(h, px, py, encoding=enc)
     d = _image_axes(scatter, nx, ny, projector)
     d.Intensity = intensity.to(h.device)
     return d, pupil.to(h.device)

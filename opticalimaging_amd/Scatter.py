@@ -42,7 +42,8 @@ def full_spectrum(field: torch.Tensor, px: float, py: float) -> torch.Tensor:
     return hk
 
 
-def pupil_image(field: torch.Tensor, px: float, py: float, hk: torch.Tensor | None = None):
+def pupil_image(field: torch.Tensor, px: float, py: float, hk: torch.Tensor | None = None,
+                encoding=None):
     """PupilImage of Scatter.CalculateSpectrum (Scatter.py:25-35): Hk with NaN
     outside the unit circle, cropped to +-(int(p)+3) samples about the centre.
 
@@ -56,7 +57,8 @@ def pupil_image(field: torch.Tensor, px: float, py: float, hk: torch.Tensor | No
     f, g = frequency_axes(nx, ny, px, py)
     inside = (ny // 2 - oy >= 0 and nx // 2 - ox >= 0 and oy <= ny - ny // 2 and ox <= nx - nx // 2)
     if inside and hk is None:
-        tens, kind, cin_, live = engine.field_encoding(h)
+        enc = encoding if encoding is not None else engine.field_encoding(h)
+        tens, kind, cin_, live = enc
         plan = engine.PLANS.get(ny=ny, nx=nx, k0r=-oy, wr=2 * oy, k0c=-ox, wc=2 * ox,
                                 nchan=len(live), max_batch=1, max_kernels=1)
         spec = torch.empty((1, len(live), 2 * oy, 2 * ox), dtype=torch.complex64, device=dev)
@@ -70,7 +72,7 @@ def pupil_image(field: torch.Tensor, px: float, py: float, hk: torch.Tensor | No
         ph = torch.exp(-2j * torch.pi * (kr[:, None] * (ny // 2) / ny + kc[None, :] * (nx // 2) / nx))
         win = torch.zeros((2 * oy, 2 * ox, cin), dtype=torch.complex64, device=dev)
         vals = (spec[0].to(torch.complex128) * ph[None] * (px * py)).to(torch.complex64)
-        for i, c in enumerate(live):
+        for i, c in enumerate(enc.source_channels):
             win[:, :, c] = vals[i]
         gsel = g[ny // 2 - oy: ny // 2 + oy].to(dev)
         fsel = f[nx // 2 - ox: nx // 2 + ox].to(dev)
@@ -82,8 +84,7 @@ def pupil_image(field: torch.Tensor, px: float, py: float, hk: torch.Tensor | No
         fsel = f[(nx // 2) - ox:(nx // 2) + ox].to(dev)
     gm, fm = torch.meshgrid(gsel, fsel, indexing="ij")
     outside = (gm ** 2 + fm ** 2) > 1
-    win[outside] = float("nan")
-    return win
+    return win.masked_fill_(outside[..., None], float("nan"))   # no host round trip
 
 
 class Scatter:

@@ -797,6 +797,13 @@ __global__ __launch_bounds__(kBlock) void k_fp_ucols(const float2* __restrict__ 
 // RES: instead of the image I, write 2 (I - target) and the workgroup's sum
 // of (I - target)^2 to part[img * gridDim.x + blockIdx.x] (the residual of a
 // least-squares fit to target, fused into the pass that produces I).
+//
+// A block images kURowSets sets of 2G rows in turn.  For Q > 0 the band tile
+// of the next set (at most Q T x 2G values) is loaded into registers while
+// the current set is transformed, so its latency hides behind the arithmetic
+// instead of opening every block.
+constexpr int kURowSets = 4;
+
 template <class P, int Q, bool RES>
 __global__ __launch_bounds__(kBlock) void k_fp_urows(const float2* __restrict__ U, int ny, int qc,
                                                      const float2* __restrict__ twt,
@@ -804,8 +811,12 @@ __global__ __launch_bounds__(kBlock) void k_fp_urows(const float2* __restrict__ 
                                                      const float* __restrict__ target,
                                                      float* __restrict__ part) {
   constexpr int N = P::N, TT = P::T, V = P::V, G = groups_per_block<P>();
-  constexpr int TS = 2 * G + 1;               // tile row stride (odd: lanes reading
+  constexpr int R = 2 * G;                    // rows per set
+  constexpr int TS = R + 1;                   // tile row stride (odd: lanes reading
                                               // consecutive q hit different banks)
+  static_assert(Q >= 0 && 2 * Q <= V, "band registers");
+  // tile values each thread prefetches (Q > 0: nq = qc + 1 <= Q T)
+  constexpr int kPre = Q > 0 ? (Q * TT * R + kBlock - 1) / kBlock : 0;
   extern __shared__ float2 lds[];
   const int nq = qc + 1;
   float2* tile = lds;                         // [nq][TS]
@@ -814,69 +825,101 @@ __global__ __launch_bounds__(kBlock) void k_fp_urows(const float2* __restrict__ 
   const bool live = g < G;
   const int gg = live ? g : 0;
   const int img = blockIdx.y;
-  const int y0 = blockIdx.x * 2 * G;
-  for (int e = threadIdx.x; e < nq * 2 * G; e += kBlock) {
-    const int q = e / (2 * G), r = e - q * 2 * G;
-    const int y = y0 + r;
-    tile[q * TS + r] = y < ny ? U[((size_t)img * nq + q) * ny + y] : make_float2(0.f, 0.f);
-  }
-  __syncthreads();
-  static_assert(Q >= 0 && 2 * Q <= V, "band registers");
-  float2 v[V];
+  const int sets = (ny + R - 1) / R;
+  const int set0 = blockIdx.x * kURowSets;
+  const int nset = min(kURowSets, sets - set0);
+  const float2* Ub = U + (size_t)img * nq * ny;
+  // tile element e = q R + r of set s: U[img][q][s R + r]
+  auto fetch = [&](int set, int e) -> float2 {
+    const int q = e / R, r = e - q * R, y = set * R + r;
+    return (q < nq && y < ny) ? Ub[(size_t)q * ny + y] : make_float2(0.f, 0.f);
+  };
+  float2 pre[kPre > 0 ? kPre : 1];
+  if constexpr (kPre > 0) {
 #pragma unroll
-  for (int m = 0; m < V; ++m) {
-    const int x = t + TT * m;
-    float2 z = make_float2(0.f, 0.f);
-    const bool may = Q == 0 || m < Q || m >= V - Q;      // compile-time for Q > 0
-    if (may && live && (x <= qc || x >= N - qc)) {
-      const bool neg = x > qc;
-      const int q = neg ? N - x : x;
-      float2 u0 = tile[q * TS + 2 * gg], u1 = tile[q * TS + 2 * gg + 1];
-      if (neg) {
-        u0.y = -u0.y;
-        u1.y = -u1.y;
-      }
-      z = make_float2(u0.x - u1.y, u0.y + u1.x);  // u0 + i u1
-    }
-    v[m] = z;
+    for (int i = 0; i < kPre; ++i) pre[i] = fetch(set0, threadIdx.x + kBlock * i);
   }
-  const int ya = y0 + 2 * gg, yb = ya + 1;
-  // residual form: the target rows are loaded before the transform, so their
-  // latency hides behind it (a load issued after the stores would also wait
-  // for them: loads and stores share one in-order counter)
-  float2 tgt[RES ? V : 1];
-  if constexpr (RES) {
+  float acc = 0.f;
+#pragma unroll 1
+  for (int s = 0; s < nset; ++s) {
+    const int set = set0 + s;
+    if constexpr (kPre > 0) {
+#pragma unroll
+      for (int i = 0; i < kPre; ++i) {
+        const int e = threadIdx.x + kBlock * i;
+        const int q = e / R, r = e - q * R;
+        if (q < nq) tile[q * TS + r] = pre[i];
+      }
+    } else {
+      for (int e = threadIdx.x; e < nq * R; e += kBlock) {
+        const int q = e / R, r = e - q * R;
+        tile[q * TS + r] = fetch(set, e);
+      }
+    }
+    __syncthreads();
+    float2 v[V];
 #pragma unroll
     for (int m = 0; m < V; ++m) {
       const int x = t + TT * m;
-      tgt[m].x = live && ya < ny ? __builtin_nontemporal_load(target + ((size_t)img * ny + ya) * N + x) : 0.f;
-      tgt[m].y = live && yb < ny ? __builtin_nontemporal_load(target + ((size_t)img * ny + yb) * N + x) : 0.f;
+      float2 z = make_float2(0.f, 0.f);
+      const bool may = Q == 0 || m < Q || m >= V - Q;      // compile-time for Q > 0
+      if (may && live && (x <= qc || x >= N - qc)) {
+        const bool neg = x > qc;
+        const int q = neg ? N - x : x;
+        float2 u0 = tile[q * TS + 2 * gg], u1 = tile[q * TS + 2 * gg + 1];
+        if (neg) {
+          u0.y = -u0.y;
+          u1.y = -u1.y;
+        }
+        z = make_float2(u0.x - u1.y, u0.y + u1.x);  // u0 + i u1
+      }
+      v[m] = z;
     }
-  }
-  P::template run_live_tab<false>(v, scr + g * P::LDS, t, twt, true);
-  if (!RES && !live) return;
-  // the image is written once and not read by this pass: non-temporal stores
-  // stream it to memory instead of leaving dirty lines in the caches
-  float acc = 0.f;
+    if constexpr (kPre > 0) {
+      if (s + 1 < nset) {
 #pragma unroll
-  for (int m = 0; m < V; ++m) {
-    const int x = t + TT * m;
-    const size_t ia = ((size_t)img * ny + ya) * N + x, ib = ((size_t)img * ny + yb) * N + x;
-    if (RES) {
-      if (live && ya < ny) {
-        const float r = v[m].x - tgt[m].x;
-        acc += r * r;
-        __builtin_nontemporal_store(2.f * r, out + ia);
+        for (int i = 0; i < kPre; ++i) pre[i] = fetch(set + 1, threadIdx.x + kBlock * i);
       }
-      if (live && yb < ny) {
-        const float r = v[m].y - tgt[m].y;
-        acc += r * r;
-        __builtin_nontemporal_store(2.f * r, out + ib);
-      }
-    } else {
-      if (ya < ny) __builtin_nontemporal_store(v[m].x, out + ia);
-      if (yb < ny) __builtin_nontemporal_store(v[m].y, out + ib);
     }
+    const int ya = set * R + 2 * gg, yb = ya + 1;
+    // residual form: the target rows are loaded before the transform, so their
+    // latency hides behind it (a load issued after the stores would also wait
+    // for them: loads and stores share one in-order counter)
+    float2 tgt[RES ? V : 1];
+    if constexpr (RES) {
+#pragma unroll
+      for (int m = 0; m < V; ++m) {
+        const int x = t + TT * m;
+        tgt[m].x = live && ya < ny ? __builtin_nontemporal_load(target + ((size_t)img * ny + ya) * N + x) : 0.f;
+        tgt[m].y = live && yb < ny ? __builtin_nontemporal_load(target + ((size_t)img * ny + yb) * N + x) : 0.f;
+      }
+    }
+    P::template run_live_tab<false>(v, scr + g * P::LDS, t, twt, true);
+    // the image is written once and not read by this pass: non-temporal stores
+    // stream it to memory instead of leaving dirty lines in the caches
+    if (live) {
+#pragma unroll
+      for (int m = 0; m < V; ++m) {
+        const int x = t + TT * m;
+        const size_t ia = ((size_t)img * ny + ya) * N + x, ib = ((size_t)img * ny + yb) * N + x;
+        if (RES) {
+          if (ya < ny) {
+            const float r = v[m].x - tgt[m].x;
+            acc += r * r;
+            __builtin_nontemporal_store(2.f * r, out + ia);
+          }
+          if (yb < ny) {
+            const float r = v[m].y - tgt[m].y;
+            acc += r * r;
+            __builtin_nontemporal_store(2.f * r, out + ib);
+          }
+        } else {
+          if (ya < ny) __builtin_nontemporal_store(v[m].x, out + ia);
+          if (yb < ny) __builtin_nontemporal_store(v[m].y, out + ib);
+        }
+      }
+    }
+    __syncthreads();                          // the tile is rewritten by the next set
   }
   if constexpr (RES) {
     __shared__ float red[kBlock / 64];
@@ -1012,7 +1055,7 @@ template <class P>
 static int run_urows(const FpArgs& a, hipStream_t st) {
   constexpr int G = groups_per_block<P>();
   const size_t lds = rows_lds<P>() + (size_t)(a.qc + 1) * (2 * G + 1) * sizeof(float2);
-  dim3 grid((a.ny + 2 * G - 1) / (2 * G), a.batch);
+  dim3 grid(fp_urows_blocks(a.ny, P::N), a.batch);
   auto go = [&](auto kernel) -> int {
     int rc = allow(kernel, lds);
     if (rc) return rc;
@@ -1192,8 +1235,11 @@ int fp_band_image(const FpArgs& a, hipStream_t st) {
 }
 
 int fp_urows_blocks(int ny, int nx) {
-#define X(NN, PT) \
-  if (nx == NN) return (ny + 2 * fp::groups_per_block<fp::PT>() - 1) / (2 * fp::groups_per_block<fp::PT>());
+#define X(NN, PT)                                                                            \
+  if (nx == NN) {                                                                          \
+    const int sets = (ny + 2 * fp::groups_per_block<fp::PT>() - 1) / (2 * fp::groups_per_block<fp::PT>()); \
+    return (sets + fp::kURowSets - 1) / fp::kURowSets;                                     \
+  }
   OI_IMAGE_PLANS(X)
 #undef X
   return 0;

@@ -85,6 +85,46 @@ __device__ __forceinline__ void tile_pair(int t, int nt, int& I, int& J) {
   J = I + rem;
 }
 
+// Tile order of the split-bf16 contraction.  Workgroups are dispatched to the
+// 8 XCDs round-robin (workgroup b runs on XCD b mod 8), and each XCD has its
+// own L2.  XCD x takes the column tiles J = 8k + x (k even) and 8k + 7 - x
+// (k odd), each with all its row tiles I = 0 .. J in turn: the operand panels
+// of the current J stay in that XCD's L2 while the I panels stream past, and
+// the zig-zag gives every XCD the same number of tiles when 8 divides ntb.
+constexpr int kXcds = 8;
+
+__host__ __device__ inline int xcd_column(int x, int k) { return (k & 1) ? kXcds * k + kXcds - 1 - x : kXcds * k + x; }
+
+__device__ __forceinline__ bool xcd_tile(int b, int ntb, int& I, int& J) {
+  const int x = b % kXcds;
+  int rem = b / kXcds;
+  for (int k = 0; kXcds * k < ntb; ++k) {
+    const int j = xcd_column(x, k);
+    if (j >= ntb) continue;
+    if (rem <= j) {
+      I = rem;
+      J = j;
+      return true;
+    }
+    rem -= j + 1;
+  }
+  return false;
+}
+
+// Workgroups of the XCD-ordered grid: 8 x the largest per-XCD tile count.
+static int xcd_tile_blocks(int ntb) {
+  int most = 0;
+  for (int x = 0; x < kXcds; ++x) {
+    int n = 0;
+    for (int k = 0; kXcds * k < ntb; ++k) {
+      const int j = xcd_column(x, k);
+      if (j < ntb) n += j + 1;
+    }
+    most = n > most ? n : most;
+  }
+  return kXcds * most;
+}
+
 template <typename T, int V>
 __device__ __forceinline__ void load_frag(const T* __restrict__ P, int M, int Kp, int row, int k,
                                           T* out) {
@@ -194,7 +234,7 @@ __global__ __launch_bounds__(256) void k_tcc_bf16(const unsigned short* __restri
   unsigned short* pan = reinterpret_cast<unsigned short*>(lds_raw);
   const size_t plane = (size_t)M * Kp;
   int I, J;
-  tile_pair(blockIdx.x, nt, I, J);
+  if (!xcd_tile(blockIdx.x, nt, I, J)) return;  // uniform: this XCD's share is done
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int ar = (wave >> 1) * (kTileB / 2), bc = (wave & 1) * (kTileB / 2);
   const int r0 = I * kTileB + ar, c0 = J * kTileB + bc;
@@ -465,7 +505,7 @@ extern "C" int oi_tcc_build(const void* G, const float* w, int M, int nsrc, int 
   float2* out = static_cast<float2*>(tcc);
   if (mode == 0) {
     const int ntb = (M + kTileB - 1) / kTileB;
-    hipLaunchKernelGGL(k_tcc_bf16, dim3(ntb * (ntb + 1) / 2), dim3(256), 0, st,
+    hipLaunchKernelGGL(k_tcc_bf16, dim3(xcd_tile_blocks(ntb)), dim3(256), 0, st,
                        static_cast<const unsigned short*>(work), M, Kp, ntb, alpha, out);
   } else if (mode == 1)
     hipLaunchKernelGGL(k_tcc_f32, dim3(pairs), dim3(256), 0, st, static_cast<const float*>(work), M,

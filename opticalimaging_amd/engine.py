@@ -328,7 +328,20 @@ class _PlanCache:
 PLANS = _PlanCache()
 
 
-def field_encoding(field):
+class FieldEncoding(tuple):
+    """(tensor, kind, cin, chans) as the C-ABI takes them; ``source_channels``
+    are the ScatterField channels that chans[i] stands for (they differ when
+    one real channel is carried as a mask)."""
+    source_channels: list
+
+
+def _encoding(parts, source_channels) -> FieldEncoding:
+    e = FieldEncoding(parts)
+    e.source_channels = list(source_channels)
+    return e
+
+
+def field_encoding(field) -> FieldEncoding:
     """Choose the cheapest device encoding of a ScatterField (Ny, Nx, C), or a
     batch of them (B, Ny, Nx, C): returns (tensor, kind, cin, live channels).
     Channels that are identically zero contribute nothing to any image
@@ -355,8 +368,8 @@ def field_encoding(field):
     if len(live) == 1 and not stats[1][live[0]]:
         h = f[..., live[0]]
         h = h.real if torch.is_complex(h) else h
-        return (h.to(torch.float32).contiguous(), nat.OI_FIELD_REAL, 1, [0])
-    return f.to(torch.complex64).contiguous(), nat.OI_FIELD_C64_CHLAST, cin, live
+        return _encoding((h.to(torch.float32).contiguous(), nat.OI_FIELD_REAL, 1, [0]), live)
+    return _encoding((f.to(torch.complex64).contiguous(), nat.OI_FIELD_C64_CHLAST, cin, live), live)
 
 
 def plan_for(ks: KernelSet, nchan: int = 1, max_batch: int = 1) -> Plan:

@@ -33,12 +33,16 @@ def focus_values(projector):
 @dataclass
 class SourceSet:
     """Lit source points on the device (float64 holding the float32 values the
-    reference selects) with per-point polarisation."""
+    reference selects) with per-point polarisation.  The extents and the
+    total weight are kept as host numbers, taken before the upload, so that
+    window sizing and normalisation need no device-to-host transfer."""
     X: torch.Tensor
     Y: torch.Tensor
     V: torch.Tensor
     PX: torch.Tensor  # c128
     PY: torch.Tensor
+    extent: tuple = None       # (xmin, xmax, ymin, ymax)
+    total: float = None        # sum of V
 
     @property
     def n(self) -> int:
@@ -46,7 +50,12 @@ class SourceSet:
 
     @property
     def weight(self) -> float:
-        return float(self.V.sum())
+        return float(self.V.sum()) if self.total is None else self.total
+
+    def bounds(self):
+        if self.extent is None:
+            return float(self.X.min()), float(self.X.max()), float(self.Y.min()), float(self.Y.max())
+        return self.extent
 
 
 def source_set(source) -> SourceSet:
@@ -70,7 +79,9 @@ def source_set(source) -> SourceSet:
         PX, PY = source.Calc_PolarizationMap(th, rho)
         PX = torch.as_tensor(PX).to(torch.complex128)
         PY = torch.as_tensor(PY).to(torch.complex128)
-    return SourceSet(X.to(dev), Y.to(dev), V.to(dev), PX.to(dev).contiguous(), PY.to(dev).contiguous())
+    extent = (float(X.min()), float(X.max()), float(Y.min()), float(Y.max()))
+    return SourceSet(X.to(dev), Y.to(dev), V.to(dev), PX.to(dev).contiguous(), PY.to(dev).contiguous(),
+                     extent, float(V.sum()))
 
 
 def optics_record(source, projector, focus, pitch_r, pitch_c, vector) -> nat.Optics:
@@ -122,8 +133,7 @@ def tcc_halfwidths(xypitch, NA, wavelength, numerics=None):
 def abbe_window(src: SourceSet, p_r: float, p_c: float, ny: int, nx: int):
     """Spectrum window covering every shifted pupil |kappa/p + s| <= 1, clipped
     to the grid's centred index range, and the per-pupil support extents."""
-    xmin, xmax = float(src.X.min()), float(src.X.max())
-    ymin, ymax = float(src.Y.min()), float(src.Y.max())
+    xmin, xmax, ymin, ymax = src.bounds()
     k0r = math.floor(p_r * (-1.0 - xmax)) - 1
     k1r = math.ceil(p_r * (1.0 - xmin)) + 1
     k0c = math.floor(p_c * (-1.0 - ymax)) - 1

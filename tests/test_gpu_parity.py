@@ -140,6 +140,28 @@ def test_abbe_vector_small(gpu):
     assert err < 1e-5
 
 
+def test_abbe_pupil_single_real_channel_not_first(gpu):
+    """One real live channel at index 1 (channels 0 and 2 zero) travels as a
+    float32 mask; the PupilImage must carry it in channel 1, NaN outside the
+    pupil, as Scatter.py:25-35 builds it."""
+    from opticalimaging_amd import workloads, CalculateAbbeImage
+    mask, src, proj, num, period = workloads.scalar_abbe_128()
+    field = np.zeros(mask.shape + (3,), dtype=np.complex64)
+    field[:, :, 1] = mask
+    sc = scatter(field, period, period)
+    img, pupil = CalculateAbbeImage(src, sc, proj, num)
+    ref, *_, pupil_o = O.abbe_image(src, sc, proj, num)
+    assert rel_err(_np(img.Intensity), ref) < 1e-5
+    pn = _np(pupil)
+    assert pn.shape == pupil_o.shape
+    assert np.array_equal(np.isnan(pn), np.isnan(pupil_o))
+    fin = ~np.isnan(pupil_o)
+    scale = np.abs(pupil_o[fin]).max()
+    assert np.abs(pn[fin] - pupil_o[fin]).max() < 1e-5 * scale
+    assert np.abs(pn[:, :, 1][fin[:, :, 1]]).max() > 0.1 * scale
+    assert np.all(pn[:, :, 0][fin[:, :, 0]] == 0) and np.all(pn[:, :, 2][fin[:, :, 2]] == 0)
+
+
 def test_abbe_complex_field_nonsquare(gpu):
     src, proj, num = annular(P=11), projector(), numerics()
     field = complex_field(48, 32, seed=21)

@@ -33,8 +33,9 @@ int make_twiddles(const int* sizes, int count, FftDesc* descs, TwiddlePool& pool
     for (int u : uniq) seen |= (u == sizes[i]);
     if (!seen) uniq.push_back(sizes[i]);
   }
+  // each length n takes 2n entries (the layout fp_stage_radices describes)
   size_t total = 0;
-  for (int u : uniq) total += (size_t)u;
+  for (int u : uniq) total += 2 * (size_t)u;
   std::vector<float2> host(total);
   std::vector<size_t> off(uniq.size());
   size_t pos = 0;
@@ -46,7 +47,11 @@ int make_twiddles(const int* sizes, int count, FftDesc* descs, TwiddlePool& pool
       const double ang = 2.0 * M_PI * (double)m / (double)n;
       host[pos + m] = make_float2((float)std::cos(ang), (float)-std::sin(ang));
     }
-    pos += (size_t)n;
+    int r1 = 0, r2 = 0;
+    const bool two = fp_stage_radices(n, &r1, &r2);
+    for (int e = 0; e < n; ++e)
+      host[pos + n + e] = two ? host[pos + (e / r1) * (e % r1)] : make_float2(0.f, 0.f);
+    pos += 2 * (size_t)n;
   }
   float2* dev = nullptr;
   if (hipMalloc(&dev, total * sizeof(float2)) != hipSuccess)
@@ -713,14 +718,22 @@ void* g_scratch_dev[kMaxDevices] = {};
 
 static __device__ __align__(256) unsigned char scratch[kNoAllocScratch];
 
-// exp(-2 pi i m / n) in float64, rounded once (the host tables of
-// make_twiddles, computed on the device).
-__global__ void k_twiddle_table(float2* __restrict__ tw, int n) {
-  const int m = blockIdx.x * blockDim.x + threadIdx.x;
-  if (m < n) {
-    const double ang = 2.0 * M_PI * (double)m / (double)n;
-    tw[m] = make_float2((float)cos(ang), (float)-sin(ang));
+// The 2n-entry tables of make_twiddles computed on the device: exp(-2 pi i
+// m / n) in float64, rounded once, for m < n, then the stage-2 entries
+// n + j r1 + k = the value at m = j k (r1 = 0: no two-stage transform, zeros).
+__global__ void k_twiddle_table(float2* __restrict__ tw, int n, int r1) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= 2 * n) return;
+  int m = e;
+  if (e >= n) {
+    if (r1 == 0) {
+      tw[e] = make_float2(0.f, 0.f);
+      return;
+    }
+    m = ((e - n) / r1) * ((e - n) % r1);
   }
+  const double ang = 2.0 * M_PI * (double)m / (double)n;
+  tw[e] = make_float2((float)cos(ang), (float)-sin(ang));
 }
 
 // Device address of `scratch` on the current GPU (looked up once per device).
@@ -775,7 +788,7 @@ extern "C" int oi_socs_image_noalloc(const oi_geometry* geo, const void* field, 
     tw_off[i] = used;
     for (int j = 0; j < i; ++j)
       if (sizes[j] == sizes[i]) tw_off[i] = tw_off[j];
-    if (tw_off[i] == used) used += align_up((size_t)sizes[i] * sizeof(float2));
+    if (tw_off[i] == used) used += align_up(2 * (size_t)sizes[i] * sizeof(float2));
   }
   const size_t per_image = aligned_total(scratch_layout(g, 1));
   if (used + per_image > kNoAllocScratch)
@@ -798,8 +811,10 @@ extern "C" int oi_socs_image_noalloc(const oi_geometry* geo, const void* field, 
     bool first = true;
     for (int j = 0; j < i; ++j) first &= (sizes[j] != sizes[i]);
     if (first) {
-      hipLaunchKernelGGL(k_twiddle_table, dim3((sizes[i] + 255) / 256), dim3(256), 0, st,
-                         reinterpret_cast<float2*>(base + tw_off[i]), sizes[i]);
+      int r1 = 0, r2 = 0;
+      if (!fp_stage_radices(sizes[i], &r1, &r2)) r1 = 0;
+      hipLaunchKernelGGL(k_twiddle_table, dim3((2 * sizes[i] + 255) / 256), dim3(256), 0, st,
+                         reinterpret_cast<float2*>(base + tw_off[i]), sizes[i], r1);
       OI_LAUNCH_CHECK("twiddle_table");
     }
   }

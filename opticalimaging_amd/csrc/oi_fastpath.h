@@ -47,6 +47,12 @@ struct FpCoh {
 
 bool fp_image_size_ok(int n);
 bool fp_coarse_size_ok(int l);
+// Radices (R1, R2) of the two-stage register transform of length n, false
+// when n has none.  Twiddle tables of length n carry 2n entries: [0, n) holds
+// exp(-2 pi i m / n) and [n, 2n) the stage-2 twiddles of that transform,
+// entry n + j R1 + k = exp(-2 pi i j k / n) (j < R2, k < R1), so that the
+// lanes of a transform group read consecutive entries.
+bool fp_stage_radices(int n, int* r1, int* r2);
 int fp_coarse_choose(int need);          // smallest table size >= need, 0 if none
 size_t fp_coherent_lds(int L, int wr, int wc);
 int fp_coherent_rows_fit(int L);

@@ -169,7 +169,7 @@ __device__ __forceinline__ void narrow_inverse(const float2 (&v)[P::V], float2* 
   for (int j = 0; j < V; ++j) {
     float2 b = scr[P::xi(t + j * R1)];
     if (j > 0) {
-      const float2 w = twt[j * t];                  // exp(-2 pi i j t / N), used conjugated
+      const float2 w = twt[P::N + j * R1 + t];      // exp(-2 pi i j t / N), used conjugated
       b = reg::mul(b, make_float2(w.x, -w.y));
     }
     sx += b.x;
@@ -1243,6 +1243,34 @@ int fp_urows_blocks(int ny, int nx) {
   OI_IMAGE_PLANS(X)
 #undef X
   return 0;
+}
+
+bool fp_stage_radices(int n, int* r1, int* r2) {
+#define X(NN, PT)                                                                            \
+  if (n == NN && fp::PT::STAGES == 2) {                                                    \
+    *r1 = fp::PT::R1;                                                                      \
+    *r2 = fp::PT::R2;                                                                      \
+    return true;                                                                           \
+  }
+  OI_IMAGE_PLANS(X)
+#undef X
+#define X(NN, A, B, NT)                                                                      \
+  if (n == NN) {                                                                           \
+    *r1 = A;                                                                               \
+    *r2 = B;                                                                               \
+    return true;                                                                           \
+  }
+  OI_COARSE_PLANS(X)
+#undef X
+#define X(NN, A, B)                                                                          \
+  if (n == NN) {                                                                           \
+    *r1 = A;                                                                               \
+    *r2 = B;                                                                               \
+    return true;                                                                           \
+  }
+  OI_TWO_PASS_PLANS(X)
+#undef X
+  return false;
 }
 
 bool fp_image_size_ok(int n) {

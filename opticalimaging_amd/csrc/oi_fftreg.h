@@ -280,12 +280,15 @@ struct GFFT {
     run_impl<INV>(v, [lds](int i) -> float2& { return lds[xi(i)]; }, t, tw, live);
   }
 
-  // Twiddles read from a table at use (table[m] = exp(-2 pi i m / N), in LDS
-  // or global memory) instead of registers.
+  // Twiddles read from a table at use instead of registers.  The table holds
+  // 2N entries: table[m] = exp(-2 pi i m / N) for m < N, then the stage-2
+  // twiddles w_N^{j k} at table[N + j R1 + k] (j < R2, k < R1), the same
+  // values laid out so that the consecutive k of a group's lanes read
+  // consecutive entries instead of a stride-j gather.
   struct TabTw {
     const float2* table;
     __device__ __forceinline__ float2 get(int u, int j, int t) const {
-      return table[j * ((t + T * u) % R1)];
+      return table[N + j * R1 + (t + T * u) % R1];
     }
   };
   struct RegTw {

@@ -118,6 +118,17 @@ size_t oi_plan_bytes(const oi_plan* plan);
 int oi_plan_set_timing(oi_plan* plan, int slots);
 int oi_plan_read_timing(oi_plan* plan, double* stage_ms, int* ncalls);
 
+/* Which channels of a field carry anything: for npix pixels of cin channels
+ * (OI_FIELD_REAL float32 [npix][cin] or OI_FIELD_C64_CHLAST c64
+ * [npix][cin], cin <= 16), ORs into the device word *mask bit 2c when
+ * channel c holds a non-zero value and bit 2c + 1 when it holds a non-zero
+ * imaginary part (*mask must be zero on entry).  Lets a caller skip the
+ * channels that contribute nothing to an image before planning, with one
+ * pass over the field instead of a reduction per channel; the reference
+ * transforms every channel (CalculateAerialImage.py:76-87).              */
+int oi_field_flags(const void* field, int kind, long long npix, int cin, unsigned int* mask,
+                   void* stream);
+
 /* Windowed spectrum of a batch of fields.
  * Replaces the spectrum step of Scatter.CalculateSpectrum (Scatter.py:10-23)
  * as used inside CalculateAerialImage.py:27 and CalculateAerialImage_SOCS.py:4.
@@ -319,6 +330,14 @@ size_t oi_ilt_work_floats(int batch);
  * and oi_ilt_residual there.  Calls on one plan share its scratch.         */
 int oi_socs_residual(oi_plan* plan, const float* mask, int batch, const void* kern, int nkern,
                      const float* target, float* resid2, float* loss, void* stream);
+/* oi_socs_residual followed by oi_socs_adjoint with grad_image = resid2:
+ * the gradient step's forward and backward passes sharing the mask spectrum
+ * (one spectrum pass instead of two).  work as for oi_socs_adjoint; the
+ * results equal the two separate calls bit for bit.  OI_ERR_UNSUPPORTED as
+ * oi_socs_residual (nothing launched).                                     */
+int oi_socs_residual_adjoint(oi_plan* plan, const float* mask, int batch, const void* kern,
+                             int nkern, const float* target, float* resid2, float* loss,
+                             void* work, float* grad_mask, void* stream);
 int oi_ilt_residual(const float* image, const float* target, float* resid2, float* loss,
                     int batch, size_t npix, float* work, void* stream);
 int oi_ilt_update(float* theta, const float* grad_m, const float* m, float lr, int batch,

@@ -71,12 +71,13 @@ def ilt(theta: torch.Tensor, target: torch.Tensor, kernels, steps: int = 50, lr:
     per-step losses tensor on the device).
 
     The gradient is the one ``ilt_loss`` gives through autograd, written out
-    so that each step is four device passes: d = 2 (I - T) and the loss with
-    I = SOCS(m) (oi_socs_residual, or oi_socs_image then oi_ilt_residual
-    where the plan has no fused form); dL/dm = adjoint(m, d); then dL/dtheta =
-    dL/dm m (1 - m), its per-image RMS, the update of theta and the next
-    m = sigmoid(theta) (oi_ilt_update).  The loop stays on the device (no
-    host synchronisation per step)."""
+    so that each step is two library calls: d = 2 (I - T), the loss with
+    I = SOCS(m) and dL/dm = adjoint(m, d) in one (oi_socs_residual_adjoint,
+    which computes the mask spectrum once for both passes; where the plan has
+    no fused image pass, oi_socs_image, oi_ilt_residual and oi_socs_adjoint);
+    then dL/dtheta = dL/dm m (1 - m), its per-image RMS, the update of theta
+    and the next m = sigmoid(theta) (oi_ilt_update).  The loop stays on the
+    device (no host synchronisation per step)."""
     ks = kernel_set(kernels)
     theta = theta.detach().to(torch.float32).contiguous().clone()
     if theta.dim() != 3:
@@ -97,19 +98,23 @@ def ilt(theta: torch.Tensor, target: torch.Tensor, kernels, steps: int = 50, lr:
     g = torch.empty_like(theta)
     m = torch.sigmoid(theta)
     st = nat.stream()
+    awork = engine.adjoint_work(ks, B, plan)
     fused = True
     for i in range(steps):
-        # d = 2 (I - T) and the loss, in the pass that produces I where the
-        # plan has one; otherwise the image, then the residual pass
+        # d = 2 (I - T), the loss and g = dL/dm, with the residual in the pass
+        # that produces I and the mask spectrum shared with the adjoint where
+        # the plan has that pass; otherwise the image, the residual pass and
+        # the adjoint one after another
         if fused:
-            fused = engine.socs_residual(m, ks, target, d, losses[i:i + 1], plan=plan)
+            fused = engine.socs_residual_adjoint(m, ks, target, d, g, awork, losses[i:i + 1],
+                                                 plan=plan)
         if not fused:
             if img is None:
                 img = torch.empty_like(theta)
             engine.socs_image(m, ks, out=img, plan=plan)
             nat.check(lib.oi_ilt_residual(nat.ptr(img), nat.ptr(target), nat.ptr(d),
                                           nat.ptr(losses[i:i + 1]), B, npix, nat.ptr(work), st))
-        engine.socs_adjoint(m, d, ks, out=g, plan=plan)
+            engine.socs_adjoint(m, d, ks, out=g, plan=plan)
         nat.check(lib.oi_ilt_update(nat.ptr(theta), nat.ptr(g), nat.ptr(m), float(lr), B, npix,
                                     nat.ptr(work), nat.ptr(m), st))
     return theta, losses

@@ -64,6 +64,7 @@ SIGNATURES = {
     "oi_plan_bytes": (c_size_t, [c_void_p]),
     "oi_plan_set_timing": (c_int, [c_void_p, c_int]),
     "oi_plan_read_timing": (c_int, [c_void_p, P_double, P_int]),
+    "oi_field_flags": (c_int, [c_void_p, c_int, ctypes.c_longlong, c_int, c_void_p, c_void_p]),
     "oi_spectrum_window": (c_int, [c_void_p, c_void_p, c_int, c_int, P_int, c_int, c_void_p, c_void_p]),
     "oi_coherent_image": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int, c_void_p, c_void_p]),
     "oi_socs_image": (c_int, [c_void_p, c_void_p, c_int, c_int, P_int, c_int, c_void_p, c_int,
@@ -95,6 +96,8 @@ SIGNATURES = {
     "oi_socs_adjoint_work_bytes": (c_size_t, [c_void_p, c_int, c_int]),
     "oi_socs_adjoint": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_int, c_void_p,
                                 c_void_p, c_void_p]),
+    "oi_socs_residual_adjoint": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int, c_void_p,
+                                         c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "oi_ilt_work_floats": (c_size_t, [c_int]),
     "oi_socs_residual": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int, c_void_p, c_void_p,
                                  c_void_p, c_void_p]),

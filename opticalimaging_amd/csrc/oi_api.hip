@@ -374,6 +374,43 @@ static FpArgs fast_args(oi_plan* p, int batch, hipStream_t st) {
 static int spectrum_impl(oi_plan* p, const void* field, int kind, int cin, const int* chans,
                          int batch, void* spec, hipStream_t st);
 
+// One pass over the field; each wave ORs its lanes' bits together and one
+// lane per wave updates the mask.
+__global__ void k_field_flags(const void* __restrict__ field, int cplx, long long npix, int cin,
+                              unsigned int* __restrict__ mask) {
+  unsigned int bits = 0;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < npix; p += stride) {
+    for (int c = 0; c < cin; ++c) {
+      if (cplx) {
+        const float2 v = static_cast<const float2*>(field)[p * cin + c];
+        if (v.x != 0.f || v.y != 0.f) bits |= 1u << (2 * c);
+        if (v.y != 0.f) bits |= 2u << (2 * c);
+      } else if (static_cast<const float*>(field)[p * cin + c] != 0.f) {
+        bits |= 1u << (2 * c);
+      }
+    }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) bits |= __shfl_xor(bits, o);
+  if ((threadIdx.x & 63) == 0 && bits) atomicOr(mask, bits);
+}
+
+extern "C" int oi_field_flags(const void* field, int kind, long long npix, int cin, unsigned int* mask,
+                              void* stream) {
+  if (!field || !mask || npix < 0 || cin < 1 || cin > 16)
+    return fail(OI_ERR_INVALID, "field_flags: bad arguments (cin must be in [1, 16])");
+  if (kind != OI_FIELD_REAL && kind != OI_FIELD_C64_CHLAST)
+    return fail(OI_ERR_INVALID, "field_flags: unknown field kind");
+  if (npix == 0) return OI_OK;
+  long long blocks = (npix + 255) / 256;
+  if (blocks > 2048) blocks = 2048;
+  hipLaunchKernelGGL(k_field_flags, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, field,
+                     kind == OI_FIELD_C64_CHLAST ? 1 : 0, npix, cin, mask);
+  OI_LAUNCH_CHECK("field_flags");
+  return OI_OK;
+}
+
 extern "C" int oi_spectrum_window(oi_plan* p, const void* field, int kind, int cin,
                                   const int* chans, int batch, void* spec, void* stream) {
   int rc = check_field(p, field, kind, cin, chans, batch);
@@ -672,6 +709,17 @@ static AdjGeom adj_geom(const oi_plan* p) {
   a.flr = p->flr;
   a.flc = p->flc;
   return a;
+}
+
+extern "C" int oi_socs_residual_adjoint(oi_plan* p, const float* mask, int batch, const void* kern,
+                                        int nkern, const float* target, float* resid2, float* loss,
+                                        void* work, float* grad_mask, void* stream) {
+  if (!p || !work || !grad_mask) return fail(OI_ERR_INVALID, "socs_residual_adjoint: null argument");
+  int rc = oi_socs_residual(p, mask, batch, kern, nkern, target, resid2, loss, stream);
+  if (rc) return rc;
+  // the residual pass left this batch's mask spectrum in p->spec
+  return launch_adjoint(adj_geom(p), resid2, p->spec, batch, static_cast<const float2*>(kern), nkern,
+                        work, grad_mask, (hipStream_t)stream);
 }
 
 extern "C" size_t oi_socs_adjoint_work_bytes(const oi_plan* p, int batch, int nkern) {

@@ -974,7 +974,7 @@ static int run_rows(const FpArgs& a, hipStream_t st) {
   int narrow = 0;
   if constexpr (P::STAGES == 2) {
     if (a.kind == OI_FIELD_REAL && a.ny % (2 * G) == 0) {
-      for (int h = 1; h <= 2 && !narrow; ++h)
+      for (int h = 1; h <= 4 && !narrow; ++h)
         if (2 * h < P::V && a.k0c >= -h * P::T && a.k0c + a.wc - 1 <= h * P::T) narrow = h;
     }
   }
@@ -989,7 +989,10 @@ static int run_rows(const FpArgs& a, hipStream_t st) {
                            a.ny, a.k0c, a.wc, a.tw_x, a.T);
         return OI_OK;
       };
-      int rc = narrow == 1 ? go(k_fp_rows_real_narrow<P, 1>) : go(k_fp_rows_real_narrow<P, 2>);
+      int rc = narrow == 1   ? go(k_fp_rows_real_narrow<P, 1>)
+               : narrow == 2 ? go(k_fp_rows_real_narrow<P, 2>)
+               : narrow == 3 ? go(k_fp_rows_real_narrow<P, 3>)
+                             : go(k_fp_rows_real_narrow<P, 4>);
       if (rc) return rc;
     }
   } else if (a.kind == OI_FIELD_REAL) {
@@ -1028,6 +1031,10 @@ static int run_cols(const FpArgs& a, hipStream_t st) {
     if (a.k0r >= -P::T && a.k0r + a.wr - 1 <= P::T) return go(k_fp_cols_narrow<P, 1>);
     if (4 < P::V && a.k0r >= -2 * P::T && a.k0r + a.wr - 1 <= 2 * P::T)
       return go(k_fp_cols_narrow<P, 2>);
+    if (6 < P::V && a.k0r >= -3 * P::T && a.k0r + a.wr - 1 <= 3 * P::T)
+      return go(k_fp_cols_narrow<P, 3>);
+    if (8 < P::V && a.k0r >= -4 * P::T && a.k0r + a.wr - 1 <= 4 * P::T)
+      return go(k_fp_cols_narrow<P, 4>);
   }
   return go(k_fp_cols<P>);
 }

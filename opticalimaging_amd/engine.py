@@ -354,22 +354,34 @@ def field_encoding(field) -> FieldEncoding:
         raise ValueError("ScatterField must be (Ny, Nx, C) or (B, Ny, Nx, C)")
     if f.dim() == 3:
         f = f.unsqueeze(0)
-    cin = f.shape[-1]
-    flat = f.reshape(-1, cin)
-    # which channels are non-zero, and which carry an imaginary part: one
-    # device-to-host transfer for both
-    if torch.is_complex(f):
-        stats = torch.stack([(flat != 0).any(0), (flat.imag != 0).any(0)]).tolist()
+    cin = int(f.shape[-1])
+    cplx = torch.is_complex(f)
+    g = f.to(torch.complex64 if cplx else torch.float32).contiguous()
+    npix = g.numel() // cin
+    # which channels are non-zero, and which carry an imaginary part: one pass
+    # over the field (oi_field_flags) and one device-to-host transfer
+    if cin <= 16:
+        word = torch.zeros(1, dtype=torch.int32, device=dev)
+        nat.check(nat.load().oi_field_flags(nat.ptr(g), nat.OI_FIELD_C64_CHLAST if cplx else nat.OI_FIELD_REAL,
+                                            npix, cin, nat.ptr(word), nat.stream()))
+        bits = int(word.item()) & 0xFFFFFFFF
+        nonzero = [bool(bits >> (2 * c) & 1) for c in range(cin)]
+        imag = [bool(bits >> (2 * c + 1) & 1) for c in range(cin)]
     else:
-        stats = [(flat != 0).any(0).tolist(), [False] * cin]
-    live = [c for c in range(cin) if stats[0][c]]
+        flat = g.reshape(-1, cin)
+        stats = torch.stack([(flat != 0).any(0), (flat.imag != 0).any(0) if cplx
+                             else torch.zeros_like(flat[0], dtype=torch.bool)]).tolist()
+        nonzero, imag = stats
+    live = [c for c in range(cin) if nonzero[c]]
     if not live:
         live = [0]
-    if len(live) == 1 and not stats[1][live[0]]:
-        h = f[..., live[0]]
-        h = h.real if torch.is_complex(h) else h
-        return _encoding((h.to(torch.float32).contiguous(), nat.OI_FIELD_REAL, 1, [0]), live)
-    return _encoding((f.to(torch.complex64).contiguous(), nat.OI_FIELD_C64_CHLAST, cin, live), live)
+    if len(live) == 1 and not imag[live[0]]:
+        h = g[..., live[0]]
+        h = h.real if cplx else h
+        return _encoding((h.contiguous(), nat.OI_FIELD_REAL, 1, [0]), live)
+    if not cplx:
+        g = g.to(torch.complex64)
+    return _encoding((g, nat.OI_FIELD_C64_CHLAST, cin, live), live)
 
 
 def plan_for(ks: KernelSet, nchan: int = 1, max_batch: int = 1) -> Plan:
@@ -497,6 +509,57 @@ def socs_residual(masks: torch.Tensor, ks: KernelSet, target: torch.Tensor, out:
         return False
     nat.check(rc)
     return True
+
+
+def socs_residual_adjoint(masks: torch.Tensor, ks: KernelSet, target: torch.Tensor,
+                          resid: torch.Tensor, grad: torch.Tensor, work: torch.Tensor,
+                          loss: torch.Tensor | None = None, plan: Plan | None = None) -> bool:
+    """resid = 2 (I - target), loss[0] = sum (I - target)^2 and grad = the
+    mask gradient of sum(resid * I), for real masks [B, ny, nx], sharing the
+    mask spectrum between the two passes (oi_socs_residual_adjoint; equal to
+    socs_residual then socs_adjoint).  work: uint8 device tensor of at least
+    oi_socs_adjoint_work_bytes(plan, B, K) bytes.  Returns False, with
+    nothing computed, when the plan has no compile-time-size image pass."""
+    if not isinstance(masks, torch.Tensor) or masks.dim() != 3:
+        raise ValueError("masks must be a (B, ny, nx) device tensor")
+    B = int(masks.shape[0])
+    shape = (B, ks.ny, ks.nx)
+    if (int(masks.shape[1]), int(masks.shape[2])) != (ks.ny, ks.nx):
+        raise ValueError(f"mask grid does not match kernel grid {ks.ny}x{ks.nx}")
+    for name, t in (("masks", masks), ("target", target), ("resid", resid), ("grad", grad)):
+        _device_tensor(name, t, torch.float32, shape)
+    if loss is not None and not (loss.is_cuda and loss.dtype == torch.float32 and loss.numel() >= 1
+                                 and loss.is_contiguous()):
+        raise ValueError("loss must be a contiguous float32 device tensor with one element")
+    if B == 0:
+        if loss is not None:
+            loss.zero_()
+        return True
+    ks = _real_set(ks, True)
+    if plan is None:
+        plan = plan_for(ks, 1, B)
+    _check_plan(plan, ks, 1, B)
+    lib = nat.load()
+    need = int(lib.oi_socs_adjoint_work_bytes(plan.handle, B, ks.K))
+    if not (isinstance(work, torch.Tensor) and work.is_cuda and work.dtype == torch.uint8
+            and work.is_contiguous() and work.numel() >= need):
+        raise ValueError(f"work must be a contiguous uint8 device tensor of at least {need} bytes")
+    rc = lib.oi_socs_residual_adjoint(plan.handle, nat.ptr(masks), B, nat.ptr(ks.win), ks.K,
+                                      nat.ptr(target), nat.ptr(resid), nat.ptr(loss), nat.ptr(work),
+                                      nat.ptr(grad), nat.stream())
+    if rc == nat.OI_ERR_UNSUPPORTED:
+        return False
+    nat.check(rc)
+    return True
+
+
+def adjoint_work(ks: KernelSet, batch: int, plan: Plan | None = None) -> torch.Tensor:
+    """Device scratch for socs_adjoint / socs_residual_adjoint on `plan`."""
+    ks = _real_set(ks, True)
+    if plan is None:
+        plan = plan_for(ks, 1, batch)
+    n = int(nat.load().oi_socs_adjoint_work_bytes(plan.handle, batch, ks.K))
+    return torch.empty(max(n, 1), dtype=torch.uint8, device=nat.device())
 
 
 def socs_adjoint(masks: torch.Tensor, grad_image: torch.Tensor, ks: KernelSet,

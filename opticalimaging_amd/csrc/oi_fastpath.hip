@@ -128,6 +128,8 @@ __device__ __forceinline__ void rows_separate(float2 zk, float2 zm, float2* dst)
 // sum_j b_j e^{+2 pi i j m / V}, the pair (m, V - m) from one set of sums.
 //   pos[m] = X[t + T m]        (m = 0 .. H)
 //   neg[i] = X[t + T (V-1-i)]  (i = 0 .. H-1), i.e. kappa = t - T (i + 1)
+// tw2: the stage-2 twiddles, w_N^{j t} at tw2[j R1 + t] (the second half of
+// a twiddle table, or a copy of it in LDS).
 // Every thread of the block calls (barrier).
 // cos and sin of 2 pi j (p + 1) / V for H > 1 output pairs, a constexpr
 // object so that no series is left for the device to evaluate (for H = 1 the
@@ -146,7 +148,7 @@ struct NarrowRoots {
 
 template <class P, int H>
 __device__ __forceinline__ void narrow_inverse(const float2 (&v)[P::V], float2* scr, int t,
-                                               const float2* __restrict__ twt,
+                                               const float2* __restrict__ tw2,
                                                float2 (&pos)[H + 1], float2 (&neg)[H]) {
   constexpr int TT = P::T, V = P::V, R1 = P::R1, U1 = P::U1;
   static_assert(P::U2 == 1 && P::R2 == V && TT == R1, "radix-V second stage");
@@ -169,7 +171,7 @@ __device__ __forceinline__ void narrow_inverse(const float2 (&v)[P::V], float2* 
   for (int j = 0; j < V; ++j) {
     float2 b = scr[P::xi(t + j * R1)];
     if (j > 0) {
-      const float2 w = twt[P::N + j * R1 + t];      // exp(-2 pi i j t / N), used conjugated
+      const float2 w = tw2[j * R1 + t];             // exp(-2 pi i j t / N), used conjugated
       b = reg::mul(b, make_float2(w.x, -w.y));
     }
     sx += b.x;
@@ -202,6 +204,20 @@ __device__ __forceinline__ void narrow_inverse(const float2 (&v)[P::V], float2* 
 // Orders kappa in [-H T, H T] of two real rows per transform.  The Hermitian
 // partner Z[-kappa] of kappa = t + T m (t > 0) is output V - 1 - m of lane
 // T - t, read across lanes; for t = 0 it is output V - m of the same lane.
+// LDS: the exchange scratch of the G groups, which the output tile [wc][2G+1]
+// reuses once every group is past its stage-2 reads, then the N stage-2
+// twiddles, copied in once per block (rows_narrow_lds).
+template <class P>
+constexpr size_t rows_narrow_scratch(int wc) {
+  return (size_t)groups_per_block<P>() * P::LDS > (size_t)wc * (2 * groups_per_block<P>() + 1)
+             ? (size_t)groups_per_block<P>() * P::LDS
+             : (size_t)wc * (2 * groups_per_block<P>() + 1);
+}
+template <class P>
+constexpr size_t rows_narrow_lds(int wc) {
+  return (rows_narrow_scratch<P>(wc) + P::N) * sizeof(float2);
+}
+
 template <class P, int H>
 __global__ __launch_bounds__(kBlock) void k_fp_rows_real_narrow(const float* __restrict__ mask,
                                                                 int ny, int k0c, int wc,
@@ -217,7 +233,9 @@ __global__ __launch_bounds__(kBlock) void k_fp_rows_real_narrow(const float* __r
   const int g = threadIdx.x / TT, t = threadIdx.x - g * TT;
   float2* scr = lds + g * P::LDS;
   constexpr int TS = 2 * G + 1;
-  float2* tile = lds + G * P::LDS;                 // [wc][TS]
+  float2* tile = lds;                              // [wc][TS], over the scratch
+  float2* tw2 = lds + rows_narrow_scratch<P>(wc);  // stage-2 twiddles [V][R1]
+  for (int e = threadIdx.x; e < N; e += kBlock) tw2[e] = twt[N + e];
   const int img = blockIdx.y;
   const float* src = mask + (size_t)img * ny * N;
   float2* Tb = T + (size_t)img * wc * ny;
@@ -234,7 +252,8 @@ __global__ __launch_bounds__(kBlock) void k_fp_rows_real_narrow(const float* __r
     for (int m = 0; m < V; ++m) v[m] = nxt[m];
     if (it + 1 < kRowSets && set + 1 < sets) rows_load_pair<P>(nxt, src, 2 * ((set + 1) * G + g), t);
     float2 pos[H + 1], neg[H];
-    narrow_inverse<P, H>(v, scr, t, twt, pos, neg);
+    narrow_inverse<P, H>(v, scr, t, tw2, pos, neg);   // its first barrier also orders tw2
+    __syncthreads();                               // every group is past its scratch reads
 #pragma unroll
     for (int m = 0; m < H; ++m) {
       // kappa = t + T m; partner: lane T - t's neg[m], or (t = 0) pos[0] / neg[m - 1]
@@ -255,7 +274,7 @@ __global__ __launch_bounds__(kBlock) void k_fp_rows_real_narrow(const float* __r
       const int j = e / (2 * G), rr = e - j * 2 * G;
       Tb[(size_t)j * ny + rb + rr] = tile[j * TS + rr];
     }
-    __syncthreads();                               // tile and scratch are reused
+    __syncthreads();                               // the tile (scratch) is reused
   }
 }
 
@@ -349,7 +368,7 @@ __global__ __launch_bounds__(kBlock) void k_fp_cols_narrow(const float2* __restr
 #pragma unroll
   for (int m = 0; m < V; ++m) v[m] = col ? src[t + TT * m] : make_float2(0.f, 0.f);
   float2 pos[H + 1], neg[H];
-  narrow_inverse<P, H>(v, lds + g * P::LDS, t, twt, pos, neg);
+  narrow_inverse<P, H>(v, lds + g * P::LDS, t, twt + N, pos, neg);
   if (!col) return;
   float2* dst = spec + (size_t)bc * wr * wc + j;
 #pragma unroll
@@ -982,10 +1001,11 @@ static int run_rows(const FpArgs& a, hipStream_t st) {
     if constexpr (P::STAGES == 2) {
       const int sets = (a.ny + 2 * G - 1) / (2 * G);
       dim3 grid((sets + kRowSets - 1) / kRowSets, a.batch);
+      const size_t lds_n = rows_narrow_lds<P>(a.wc);
       auto go = [&](auto kernel) -> int {
-        int rc = allow(kernel, lds);
+        int rc = allow(kernel, lds_n);
         if (rc) return rc;
-        hipLaunchKernelGGL(kernel, grid, dim3(kBlock), lds, st, static_cast<const float*>(a.field),
+        hipLaunchKernelGGL(kernel, grid, dim3(kBlock), lds_n, st, static_cast<const float*>(a.field),
                            a.ny, a.k0c, a.wc, a.tw_x, a.T);
         return OI_OK;
       };

@@ -190,7 +190,9 @@ __global__ __launch_bounds__(kRegBlock) void k_adj_cfield_reg(const float2* __re
     float2 a = make_float2(0.f, 0.f);
     if (on && ir < wr) {
       a = sb[(size_t)ir * wc];
-      if (kb) a = cmul(a, kb[(size_t)ir * wc]);
+      // kernel x spectrum as the forward's k_fp_bcols forms it, so that Ec
+      // holds the same bits as its column buffer
+      if (kb) a = reg::mul_rn(kb[(size_t)ir * wc], a);
     }
     v[m] = a;
   }
@@ -238,6 +240,59 @@ __global__ __launch_bounds__(kRegBlock) void k_adj_rfield_reg(float2* __restrict
   }
   P::template run_live_tab<false>(v, scr, t, twt, g < G);
   if (!on) return;
+#pragma unroll
+  for (int m = 0; m < V; ++m) {
+    const int kc = pmod(t + T * m - k0c, L);
+    if (kc < wc) row[kc] = v[m];
+  }
+}
+
+// k_adj_rfield_reg (with gamma) taking the column stage from the forward
+// pass's two-pass buffer Bc[b][k][kc][t] (k_fp_bcols: the same values as
+// k_adj_cfield_reg's Ec, column-major) instead of recomputing it.  A block
+// stages rows r0 .. r0 + G - 1 of every window column through LDS (one
+// contiguous run of G values per column), runs the row stage and writes the
+// result to Ec[b][k][r][kc] as k_adj_rfield_reg does.
+template <class P>
+__global__ __launch_bounds__(kRegBlock) void k_adj_rfield_bcols(const float2* __restrict__ Bc,
+                                                                float2* __restrict__ Ec, int nk,
+                                                                int k0c, int wc, int lr,
+                                                                const float2* __restrict__ twt,
+                                                                const float* __restrict__ gamma_in) {
+  constexpr int L = P::N, T = P::T, V = P::V, G = kRegBlock / T;
+  constexpr int TS = G + 1;
+  extern __shared__ float2 lds[];
+  float2* tile = lds;                              // [wc][TS]
+  float2* scr = lds + (size_t)wc * TS;
+  const int g = threadIdx.x / T, t = threadIdx.x - g * T;
+  const int gg = g < G ? g : 0;
+  const int k = blockIdx.y, b = blockIdx.z;
+  const int r0 = blockIdx.x * G;
+  const int r = r0 + gg;
+  const bool on = g < G && r < lr;
+  const float2* src = Bc + ((size_t)b * nk + k) * wc * lr + r0;
+  for (int e = threadIdx.x; e < wc * G; e += kRegBlock) {
+    const int kc = e / G, rr = e - kc * G;
+    tile[kc * TS + rr] = r0 + rr < lr ? src[(size_t)kc * lr + rr] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  float2 v[V];
+#pragma unroll
+  for (int m = 0; m < V; ++m) {
+    const int kc = pmod(t + T * m - k0c, L);
+    v[m] = (on && kc < wc) ? tile[kc * TS + gg] : make_float2(0.f, 0.f);
+  }
+  float2* sg = scr + (size_t)gg * P::LDS;
+  P::template run_live_tab<false>(v, sg, t, twt, g < G);
+  const float* gi = gamma_in + ((size_t)b * lr + (on ? r : 0)) * L;
+#pragma unroll
+  for (int m = 0; m < V; ++m) {
+    const float gm = on ? gi[t + T * m] : 0.f;
+    v[m] = make_float2(gm * v[m].x, -gm * v[m].y);
+  }
+  P::template run_live_tab<false>(v, sg, t, twt, g < G);
+  if (!on) return;
+  float2* row = Ec + (((size_t)b * nk + k) * lr + r) * wc;
 #pragma unroll
   for (int m = 0; m < V; ++m) {
     const int kc = pmod(t + T * m - k0c, L);
@@ -328,6 +383,36 @@ static int adj_rfield_reg(int L, dim3 grid_y_z, float2* Ec, int nk, int k0c, int
   OI_ADJ_PLANS(X)
 #undef X
   return fail(OI_ERR_UNSUPPORTED, "adjoint: no register transform for this coarse size");
+}
+
+// Sizes of the forward's two-pass column buffer.
+#define OI_ADJ_BCOLS_PLANS(X) X(144, 12, 12) X(256, 16, 16) X(512, 16, 32)
+
+static bool adj_bcols_size(int L) {
+#define X(NN, A, B) if (L == NN) return true;
+  OI_ADJ_BCOLS_PLANS(X)
+#undef X
+  return false;
+}
+
+static int adj_rfield_bcols(int L, dim3 grid_y_z, const float2* Bc, float2* Ec, int nk, int k0c,
+                            int wc, int lr, const float2* twt, const float* gamma_in, hipStream_t st) {
+#define X(NN, A, B)                                                                            \
+  if (L == NN) {                                                                               \
+    using P = reg::GFFT<NN, A, B>;                                                             \
+    constexpr int G = kRegBlock / P::T;                                                        \
+    const size_t lds = ((size_t)wc * (G + 1) + (size_t)G * P::LDS) * sizeof(float2);           \
+    int rc = allow_lds_adj(k_adj_rfield_bcols<P>, lds);                                        \
+    if (rc) return rc;                                                                         \
+    dim3 grid((lr + G - 1) / G, grid_y_z.y, grid_y_z.z);                                       \
+    hipLaunchKernelGGL(k_adj_rfield_bcols<P>, grid, dim3(kRegBlock), lds, st, Bc, Ec, nk, k0c, \
+                       wc, lr, twt, gamma_in);                                                 \
+    OI_LAUNCH_CHECK("adj_rfield_bcols");                                                       \
+    return OI_OK;                                                                              \
+  }
+  OI_ADJ_BCOLS_PLANS(X)
+#undef X
+  return fail(OI_ERR_UNSUPPORTED, "adjoint: no two-pass column buffer for this coarse size");
 }
 
 static int adj_cwin_reg(int L, dim3 grid_y_z, const float2* Pc, const float2* kern, int nk,
@@ -494,7 +579,8 @@ static char* carve(char*& p, size_t bytes) {
 }
 
 int launch_adjoint(const AdjGeom& g, const float* gI, const float2* spec, int batch,
-                   const float2* kern, int nkern, void* work, float* grad, hipStream_t st) {
+                   const float2* kern, int nkern, void* work, float* grad, hipStream_t st,
+                   const float2* bcols) {
   const size_t B = (size_t)batch, K = (size_t)nkern;
   char* w = static_cast<char*>(work);
   const int wgc = 2 * g.qc + 1, wgr = 2 * g.qr + 1;
@@ -574,7 +660,15 @@ int launch_adjoint(const AdjGeom& g, const float* gI, const float2* spec, int ba
   }
   const float* gam = g.direct ? gI : gamma;
 
-  if (reg) {
+  if (reg && bcols && !g.direct && g.lr == g.lc && adj_bcols_size(g.lr)) {
+    // the column stage is the forward pass's column buffer
+    if ((rc = adj_rfield_bcols(g.lc, dim3(1, nkern, batch), bcols, Ec, nkern, g.k0c, g.wc, g.lr,
+                               g.flc.tw, gam, st)))
+      return rc;
+    if ((rc = adj_cwin_reg(g.lr, dim3(1, nkern, batch), Ec, kern, nkern, g.k0r, g.wr, g.wc,
+                           g.flr.tw, Ak, st)))
+      return rc;
+  } else if (reg) {
     if ((rc = adj_cfield_reg(g.lr, dim3(1, nkern, batch), spec, kern, nkern, g.k0r, g.wr, g.wc,
                              g.flr.tw, Ec, st)))
       return rc;

@@ -717,9 +717,11 @@ extern "C" int oi_socs_residual_adjoint(oi_plan* p, const float* mask, int batch
   if (!p || !work || !grad_mask) return fail(OI_ERR_INVALID, "socs_residual_adjoint: null argument");
   int rc = oi_socs_residual(p, mask, batch, kern, nkern, target, resid2, loss, stream);
   if (rc) return rc;
-  // the residual pass left this batch's mask spectrum in p->spec
+  // the residual pass left this batch's mask spectrum in p->spec and, on the
+  // two-pass path, its kernels' column stage in p->Bbuf
+  const float2* bcols = p->g.path == 2 ? p->Bbuf : nullptr;
   return launch_adjoint(adj_geom(p), resid2, p->spec, batch, static_cast<const float2*>(kern), nkern,
-                        work, grad_mask, (hipStream_t)stream);
+                        work, grad_mask, (hipStream_t)stream, bcols);
 }
 
 extern "C" size_t oi_socs_adjoint_work_bytes(const oi_plan* p, int batch, int nkern) {

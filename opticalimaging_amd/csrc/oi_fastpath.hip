@@ -657,7 +657,8 @@ __global__ __launch_bounds__(kBlock) void k_fp_bcols(const float2* __restrict__ 
   for (int m = 0; m < VL; ++m) {
     int kr = kr0 + TL * m;
     kr = kr >= L ? kr - L : kr;
-    v[m] = (live && kr < wr) ? reg::mul(kp[kr * wc + c], sp[kr * wc + c]) : make_float2(0.f, 0.f);
+    // rounded product: the adjoint's k_adj_cfield_reg forms the same values
+    v[m] = (live && kr < wr) ? reg::mul_rn(kp[kr * wc + c], sp[kr * wc + c]) : make_float2(0.f, 0.f);
   }
   PL::template run_live_tab<false>(v, lds + g * PL::LDS, t, twt, live);
   if (!live) return;

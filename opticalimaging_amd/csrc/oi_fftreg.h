@@ -59,6 +59,14 @@ __device__ __forceinline__ float2 mul(float2 a, float2 b) {
   return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
+// The complex product with each multiply and add rounded on its own (no
+// fused multiply-add), so that two kernels forming the same product get the
+// same bits whatever the compiler fuses elsewhere in them.
+__device__ __forceinline__ float2 mul_rn(float2 a, float2 b) {
+#pragma clang fp contract(off)
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
 template <int N, bool INV>
 struct DFT;
 

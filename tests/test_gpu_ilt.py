@@ -161,10 +161,13 @@ def test_socs_residual_matches_image_then_residual(gpu, case):
 
 @pytest.mark.parametrize("case", ["one-pass 256", "two-pass 1024"])
 def test_socs_residual_adjoint_equals_separate_calls(gpu, case):
-    """oi_socs_residual_adjoint (one mask spectrum for both passes) gives the
-    residual, the loss and the mask gradient of oi_socs_residual followed by
-    oi_socs_adjoint, bit for bit; the two-pass case is the 1024^2 NA 1.35
-    workload whose cotangent band (217 orders) takes the H = 4 narrow rows."""
+    """oi_socs_residual_adjoint (one mask spectrum for both passes; on the
+    two-pass path the adjoint also takes its column stage from the forward's
+    column buffer) gives the residual and the loss of oi_socs_residual bit for
+    bit, and the mask gradient of oi_socs_adjoint to float32 rounding (the
+    same values formed by other kernels: <= 1e-6 of the gradient's peak).
+    The two-pass case is the 1024^2 NA 1.35 workload whose cotangent band
+    (217 orders) takes the H = 4 narrow rows."""
     from opticalimaging_amd import engine, workloads
     from opticalimaging_amd.DecomposeTCC_SOCS import socs_kernel_set
     if case == "one-pass 256":
@@ -191,8 +194,11 @@ def test_socs_residual_adjoint_equals_separate_calls(gpu, case):
     work = engine.adjoint_work(ks, B, plan)
     assert engine.socs_residual_adjoint(m, ks, T, d2, g2, work, l2, plan=plan)
     torch.cuda.synchronize()
-    assert torch.equal(d1, d2) and torch.equal(l1, l2) and torch.equal(g1, g2)
-    assert float(g1.abs().max()) > 0
+    assert torch.equal(d1, d2) and torch.equal(l1, l2)
+    peak = float(g1.abs().max())
+    err = float((g1 - g2).abs().max()) / peak
+    print(f"{case}: fused vs separate gradient {err:.2e} of its peak")
+    assert peak > 0 and err <= 1e-6
 
 
 def test_socs_residual_reports_generic_plans(gpu):

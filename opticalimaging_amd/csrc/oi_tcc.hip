@@ -181,25 +181,33 @@ __device__ __forceinline__ void store_frag(float2* __restrict__ tcc, int M, int 
   wave_lds_sync();                          // the buffer is rewritten by the next fragment
 }
 
-// Split-bf16 contraction: workgroup tile kTileB x kTileB (4 waves in 2 x 2,
-// each (kTileB/2)^2 as FT x FT MFMA tiles; 64 measured faster than 128 at
-// M = 10201: more workgroups per CU overlap one's stores with another's MFMAs).  Panels of a k-step (32 wide) are
-// staged in LDS: A rows (tile I) and X, Z rows (tile J), hi and lo each; rows
-// padded to 40 halves (16-byte aligned, spread over banks).  The next k-step's
-// panels are loaded into registers while this step's MFMAs run.
-constexpr int kTileB = 64;
+// Split-bf16 contraction: workgroup tile TB x TB (4 waves in 2 x 2, each
+// (TB/2)^2 as FT x FT MFMA tiles).  Panels of a k-step (32 wide) are staged in
+// LDS: A rows (tile I) and X, Z rows (tile J), hi and lo each; rows padded to
+// 40 halves (16-byte aligned, spread over banks).  The next k-step's panels
+// are loaded into registers while this step's MFMAs run.
 constexpr int kPanelK = 32, kPanelRow = 40;
 constexpr int kPanels = 6;                      // Ah Al Xh Xl Zh Zl
-constexpr int kPanelChunks = kPanels * kTileB * 4 / 256;   // 16-byte chunks per thread
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
+template <int TB>
+struct TccTile {
+  static constexpr int FT = TB / 32;                        // MFMA tiles per wave and dimension
+  static constexpr int kChunks = kPanels * TB * 4 / 256;    // 16-byte panel chunks per thread
+  static constexpr int kOutStride = TB + 1;
+  static constexpr size_t kPanBytes = (size_t)kPanels * TB * kPanelRow * sizeof(unsigned short);
+  static constexpr size_t kOutBytes = (size_t)TB * kOutStride * sizeof(float2);
+  static constexpr size_t kLds = kPanBytes > kOutBytes ? kPanBytes : kOutBytes;
+};
+
+template <int TB>
 __device__ __forceinline__ void tcc_panel_load(const unsigned short* __restrict__ planes, size_t plane,
                                                int M, int Kp, int rI, int rJ, int k0,
-                                               u32x4 (&reg)[kPanelChunks]) {
+                                               u32x4 (&reg)[TccTile<TB>::kChunks]) {
 #pragma unroll
-  for (int q = 0; q < kPanelChunks; ++q) {
+  for (int q = 0; q < TccTile<TB>::kChunks; ++q) {
     const int c = threadIdx.x + 256 * q;
-    const int panel = c / (kTileB * 4), rem = c - panel * kTileB * 4, row = rem >> 2, ch = rem & 3;
+    const int panel = c / (TB * 4), rem = c - panel * TB * 4, row = rem >> 2, ch = rem & 3;
     // panel order: A hi (X hi rows of I), A lo, X hi (rows of J), X lo, Z hi, Z lo
     const int pl = panel < 2 ? panel : panel - 2;
     const int r = (panel < 2 ? rI : rJ) + row;
@@ -208,36 +216,35 @@ __device__ __forceinline__ void tcc_panel_load(const unsigned short* __restrict_
   }
 }
 
-__device__ __forceinline__ void tcc_panel_store(unsigned short* lds, const u32x4 (&reg)[kPanelChunks]) {
+template <int TB>
+__device__ __forceinline__ void tcc_panel_store(unsigned short* lds,
+                                                const u32x4 (&reg)[TccTile<TB>::kChunks]) {
 #pragma unroll
-  for (int q = 0; q < kPanelChunks; ++q) {
+  for (int q = 0; q < TccTile<TB>::kChunks; ++q) {
     const int c = threadIdx.x + 256 * q;
-    const int panel = c / (kTileB * 4), rem = c - panel * kTileB * 4, row = rem >> 2, ch = rem & 3;
-    *reinterpret_cast<u32x4*>(lds + (panel * kTileB + row) * kPanelRow + ch * 8) = reg[q];
+    const int panel = c / (TB * 4), rem = c - panel * TB * 4, row = rem >> 2, ch = rem & 3;
+    *reinterpret_cast<u32x4*>(lds + (panel * TB + row) * kPanelRow + ch * 8) = reg[q];
   }
 }
 
-// Output tile staging: the workgroup's 64 x 64 result goes through LDS so
-// that each store instruction writes one 64-entry row segment (512 B) of the
-// TCC, and the mirrored tile one row of the conjugate transpose, instead of
-// four 128-byte pieces of four rows.  It shares the LDS of the operand panels.
-constexpr int kOutStride = kTileB + 1;
-constexpr size_t kPanBytes = (size_t)kPanels * kTileB * kPanelRow * sizeof(unsigned short);
-constexpr size_t kOutBytes = (size_t)kTileB * kOutStride * sizeof(float2);
-constexpr size_t kTccLds = kPanBytes > kOutBytes ? kPanBytes : kOutBytes;
-
+// Output tile staging: the workgroup's TB x TB result goes through LDS so
+// that each store instruction writes one TB-entry row segment of the TCC, and
+// the mirrored tile one row of the conjugate transpose, instead of four
+// 128-byte pieces of four rows.  It shares the LDS (dynamic, TccTile::kLds
+// bytes) of the operand panels.
+template <int TB>
 __global__ __launch_bounds__(256) void k_tcc_bf16(const unsigned short* __restrict__ planes, int M,
                                                   int Kp, int nt, double alpha,
                                                   float2* __restrict__ tcc) {
-  constexpr int FT = kTileB / 32;               // MFMA tiles per wave and dimension
-  __shared__ __align__(16) unsigned char lds_raw[kTccLds];
+  using TT = TccTile<TB>;
+  constexpr int FT = TT::FT, kOutStride = TT::kOutStride;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
   unsigned short* pan = reinterpret_cast<unsigned short*>(lds_raw);
   const size_t plane = (size_t)M * Kp;
   int I, J;
   if (!xcd_tile(blockIdx.x, nt, I, J)) return;  // uniform: this XCD's share is done
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int ar = (wave >> 1) * (kTileB / 2), bc = (wave & 1) * (kTileB / 2);
-  const int r0 = I * kTileB + ar, c0 = J * kTileB + bc;
+  const int ar = (wave >> 1) * (TB / 2), bc = (wave & 1) * (TB / 2);
   const int li = lane & 15, lk = (lane >> 4) * 8;
   f32x4 accR[FT][FT], accI[FT][FT];
 #pragma unroll
@@ -247,23 +254,23 @@ __global__ __launch_bounds__(256) void k_tcc_bf16(const unsigned short* __restri
       accR[a][b] = f32x4{0, 0, 0, 0};
       accI[a][b] = f32x4{0, 0, 0, 0};
     }
-  u32x4 nxt[kPanelChunks];
-  tcc_panel_load(planes, plane, M, Kp, I * kTileB, J * kTileB, 0, nxt);
+  u32x4 nxt[TT::kChunks];
+  tcc_panel_load<TB>(planes, plane, M, Kp, I * TB, J * TB, 0, nxt);
   for (int k0 = 0; k0 < Kp; k0 += kPanelK) {
     __syncthreads();                            // previous step's fragment reads are done
-    tcc_panel_store(pan, nxt);
+    tcc_panel_store<TB>(pan, nxt);
     __syncthreads();
-    if (k0 + kPanelK < Kp) tcc_panel_load(planes, plane, M, Kp, I * kTileB, J * kTileB, k0 + kPanelK, nxt);
+    if (k0 + kPanelK < Kp) tcc_panel_load<TB>(planes, plane, M, Kp, I * TB, J * TB, k0 + kPanelK, nxt);
     bf16x8 ah[FT], al[FT], rh[FT], rl[FT], ih[FT], il[FT];
 #pragma unroll
     for (int t = 0; t < FT; ++t) {
       const int ra = ar + t * 16 + li, cb = bc + t * 16 + li;
-      ah[t] = *reinterpret_cast<const bf16x8*>(pan + (0 * kTileB + ra) * kPanelRow + lk);
-      al[t] = *reinterpret_cast<const bf16x8*>(pan + (1 * kTileB + ra) * kPanelRow + lk);
-      rh[t] = *reinterpret_cast<const bf16x8*>(pan + (2 * kTileB + cb) * kPanelRow + lk);
-      rl[t] = *reinterpret_cast<const bf16x8*>(pan + (3 * kTileB + cb) * kPanelRow + lk);
-      ih[t] = *reinterpret_cast<const bf16x8*>(pan + (4 * kTileB + cb) * kPanelRow + lk);
-      il[t] = *reinterpret_cast<const bf16x8*>(pan + (5 * kTileB + cb) * kPanelRow + lk);
+      ah[t] = *reinterpret_cast<const bf16x8*>(pan + (0 * TB + ra) * kPanelRow + lk);
+      al[t] = *reinterpret_cast<const bf16x8*>(pan + (1 * TB + ra) * kPanelRow + lk);
+      rh[t] = *reinterpret_cast<const bf16x8*>(pan + (2 * TB + cb) * kPanelRow + lk);
+      rl[t] = *reinterpret_cast<const bf16x8*>(pan + (3 * TB + cb) * kPanelRow + lk);
+      ih[t] = *reinterpret_cast<const bf16x8*>(pan + (4 * TB + cb) * kPanelRow + lk);
+      il[t] = *reinterpret_cast<const bf16x8*>(pan + (5 * TB + cb) * kPanelRow + lk);
     }
 #pragma unroll
     for (int a = 0; a < FT; ++a)
@@ -291,21 +298,34 @@ __global__ __launch_bounds__(256) void k_tcc_bf16(const unsigned short* __restri
                                                   (float)(alpha * (double)accI[a][b][i]));
       }
   __syncthreads();
-  // tile (I, J): wave w writes rows w, w + 4, ...; lane = column
-  const int col = J * kTileB + lane;
-  for (int rl = wave; rl < kTileB; rl += 4) {
-    const int row = I * kTileB + rl;
-    if (row < M && col < M) tcc[(size_t)row * M + col] = otile[rl * kOutStride + lane];
+  // tile (I, J): wave w writes rows w, w + 4, ...; lanes cover the row's TB entries
+  for (int rl = wave; rl < TB; rl += 4) {
+    const int row = I * TB + rl;
+#pragma unroll
+    for (int h = 0; h < TB / 64; ++h) {
+      const int cl = lane + 64 * h, col = J * TB + cl;
+      if (row < M && col < M) tcc[(size_t)row * M + col] = otile[rl * kOutStride + cl];
+    }
   }
   if (I == J) return;                           // uniform over the workgroup
-  // tile (J, I) = conjugate transpose: row J*64 + cl holds column cl of the tile
-  const int mcol = I * kTileB + lane;
-  for (int cl = wave; cl < kTileB; cl += 4) {
-    const int row = J * kTileB + cl;
-    const float2 v = otile[lane * kOutStride + cl];
-    if (row < M && mcol < M) tcc[(size_t)row * M + mcol] = make_float2(v.x, -v.y);
+  // tile (J, I) = conjugate transpose: row J*TB + cl holds column cl of the tile
+  for (int cl = wave; cl < TB; cl += 4) {
+    const int row = J * TB + cl;
+#pragma unroll
+    for (int h = 0; h < TB / 64; ++h) {
+      const int rl = lane + 64 * h, mcol = I * TB + rl;
+      const float2 v = otile[rl * kOutStride + cl];
+      if (row < M && mcol < M) tcc[(size_t)row * M + mcol] = make_float2(v.x, -v.y);
+    }
   }
 }
+
+// Workgroup tile of the split-bf16 contraction.  At M = 10201 a 128 x 128
+// tile (64 x 64 per wave, 1 KB output row segments) ran in the same time as
+// 64 x 64 (0.43 against 0.42 ms): its 294 registers leave one workgroup per
+// CU, and the output stream, not the MFMAs, sets the pace (the contraction
+// without its stores takes 0.24 ms).
+constexpr int kTileB = 64;
 
 // fp32 MFMA 16x16x4: lane fragment is one value at k = l>>4; a 16-wide K step
 // runs four MFMAs, sub-step kk using k = k0 + 4 (l>>4) + kk for both operands.
@@ -505,7 +525,11 @@ extern "C" int oi_tcc_build(const void* G, const float* w, int M, int nsrc, int 
   float2* out = static_cast<float2*>(tcc);
   if (mode == 0) {
     const int ntb = (M + kTileB - 1) / kTileB;
-    hipLaunchKernelGGL(k_tcc_bf16, dim3(xcd_tile_blocks(ntb)), dim3(256), 0, st,
+    constexpr size_t lds = TccTile<kTileB>::kLds;
+    if (lds > 64 * 1024)
+      OI_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_tcc_bf16<kTileB>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(k_tcc_bf16<kTileB>, dim3(xcd_tile_blocks(ntb)), dim3(256), lds, st,
                        static_cast<const unsigned short*>(work), M, Kp, ntb, alpha, out);
   } else if (mode == 1)
     hipLaunchKernelGGL(k_tcc_f32, dim3(pairs), dim3(256), 0, st, static_cast<const float*>(work), M,

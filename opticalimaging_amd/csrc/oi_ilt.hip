@@ -13,7 +13,7 @@
 namespace oi {
 namespace {
 
-constexpr int kIltBlocks = 64;    // workgroups per image
+constexpr int kIltBlocks = 256;   // workgroups per image (enough loads in flight to stream)
 constexpr int kIltThreads = 256;
 
 typedef float f4v __attribute__((ext_vector_type(4)));
@@ -123,10 +123,12 @@ __global__ __launch_bounds__(kIltThreads) void k_ilt_update(float* theta, const 
                                                            const float* m, float lr, size_t npix,
                                                            const float* __restrict__ part,
                                                            float* m_next) {
-  static_assert(kIltBlocks == 64, "one partial sum per lane of the first wave");
+  static_assert(kIltBlocks % 64 == 0, "whole partial sums per lane of the first wave");
   __shared__ float coef;
   if (threadIdx.x < 64) {
-    const double s = wave_sum(part[(size_t)blockIdx.y * kIltBlocks + threadIdx.x]);
+    double s = 0.0;                             // lane l adds l, l + 64, ... in order
+    for (int i = threadIdx.x; i < kIltBlocks; i += 64) s += part[(size_t)blockIdx.y * kIltBlocks + i];
+    s = wave_sum(s);
     const double rms = sqrt(s) / sqrt((double)npix) + 1e-30;
     if (threadIdx.x == 0) coef = (float)(lr / rms);
   }

@@ -332,9 +332,11 @@ int oi_socs_residual(oi_plan* plan, const float* mask, int batch, const void* ke
                      const float* target, float* resid2, float* loss, void* stream);
 /* oi_socs_residual followed by oi_socs_adjoint with grad_image = resid2:
  * the gradient step's forward and backward passes sharing the mask spectrum
- * (one spectrum pass instead of two).  work as for oi_socs_adjoint; the
- * results equal the two separate calls bit for bit.  OI_ERR_UNSUPPORTED as
- * oi_socs_residual (nothing launched).                                     */
+ * (one spectrum pass instead of two) and, on two-pass plans (path 2), the
+ * kernels' column-stage fields of the forward pass.  work as for
+ * oi_socs_adjoint.  resid2 and loss equal oi_socs_residual's bit for bit;
+ * grad_mask equals oi_socs_adjoint's to float32 rounding.
+ * OI_ERR_UNSUPPORTED as oi_socs_residual (nothing launched).              */
 int oi_socs_residual_adjoint(oi_plan* plan, const float* mask, int batch, const void* kern,
                              int nkern, const float* target, float* resid2, float* loss,
                              void* work, float* grad_mask, void* stream);

@@ -516,8 +516,9 @@ def socs_residual_adjoint(masks: torch.Tensor, ks: KernelSet, target: torch.Tens
                           loss: torch.Tensor | None = None, plan: Plan | None = None) -> bool:
     """resid = 2 (I - target), loss[0] = sum (I - target)^2 and grad = the
     mask gradient of sum(resid * I), for real masks [B, ny, nx], sharing the
-    mask spectrum between the two passes (oi_socs_residual_adjoint; equal to
-    socs_residual then socs_adjoint).  work: uint8 device tensor of at least
+    mask spectrum (and on two-pass plans the column-stage fields) between the
+    two passes (oi_socs_residual_adjoint: resid and loss equal socs_residual's
+    bit for bit, grad equals socs_adjoint's to float32 rounding).  work: uint8 device tensor of at least
     oi_socs_adjoint_work_bytes(plan, B, K) bytes.  Returns False, with
     nothing computed, when the plan has no compile-time-size image pass."""
     if not isinstance(masks, torch.Tensor) or masks.dim() != 3:

@@ -252,7 +252,8 @@ __global__ __launch_bounds__(kRegBlock) void k_adj_rfield_reg(float2* __restrict
 // k_adj_cfield_reg's Ec, column-major) instead of recomputing it.  A block
 // stages rows r0 .. r0 + G - 1 of every window column through LDS (one
 // contiguous run of G values per column), runs the row stage and writes the
-// result to Ec[b][k][r][kc] as k_adj_rfield_reg does.
+// result to Ec[b][k][r][kc] as k_adj_rfield_reg does.  The staging tile and
+// the transform scratch share the LDS (the larger of the two).
 template <class P>
 __global__ __launch_bounds__(kRegBlock) void k_adj_rfield_bcols(const float2* __restrict__ Bc,
                                                                 float2* __restrict__ Ec, int nk,
@@ -262,8 +263,8 @@ __global__ __launch_bounds__(kRegBlock) void k_adj_rfield_bcols(const float2* __
   constexpr int L = P::N, T = P::T, V = P::V, G = kRegBlock / T;
   constexpr int TS = G + 1;
   extern __shared__ float2 lds[];
-  float2* tile = lds;                              // [wc][TS]
-  float2* scr = lds + (size_t)wc * TS;
+  float2* tile = lds;                              // [wc][TS], then the scratch over it
+  float2* scr = lds;
   const int g = threadIdx.x / T, t = threadIdx.x - g * T;
   const int gg = g < G ? g : 0;
   const int k = blockIdx.y, b = blockIdx.z;
@@ -282,6 +283,7 @@ __global__ __launch_bounds__(kRegBlock) void k_adj_rfield_bcols(const float2* __
     const int kc = pmod(t + T * m - k0c, L);
     v[m] = (on && kc < wc) ? tile[kc * TS + gg] : make_float2(0.f, 0.f);
   }
+  __syncthreads();                                 // the tile is read: the scratch reuses it
   float2* sg = scr + (size_t)gg * P::LDS;
   P::template run_live_tab<false>(v, sg, t, twt, g < G);
   const float* gi = gamma_in + ((size_t)b * lr + (on ? r : 0)) * L;
@@ -401,7 +403,8 @@ static int adj_rfield_bcols(int L, dim3 grid_y_z, const float2* Bc, float2* Ec, 
   if (L == NN) {                                                                               \
     using P = reg::GFFT<NN, A, B>;                                                             \
     constexpr int G = kRegBlock / P::T;                                                        \
-    const size_t lds = ((size_t)wc * (G + 1) + (size_t)G * P::LDS) * sizeof(float2);           \
+    const size_t tile = (size_t)wc * (G + 1), scr = (size_t)G * P::LDS;                        \
+    const size_t lds = (tile > scr ? tile : scr) * sizeof(float2);                             \
     int rc = allow_lds_adj(k_adj_rfield_bcols<P>, lds);                                        \
     if (rc) return rc;                                                                         \
     dim3 grid((lr + G - 1) / G, grid_y_z.y, grid_y_z.z);                                       \

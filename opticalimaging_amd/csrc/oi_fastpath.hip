@@ -680,7 +680,7 @@ __global__ __launch_bounds__(kBlock) void k_fp_brows(const float2* __restrict__ 
   static_assert(VL <= 32, "row mask holds VL bits");
   extern __shared__ float2 lds[];
   float2* tile = lds;                      // [wc][TS]
-  float2* scr = lds + (size_t)wc * TS;     // transform scratch
+  float2* scr = lds;                       // transform scratch, over the tile once it is read
   const int g = threadIdx.x / TL, t = threadIdx.x - g * TL;
   const int img = blockIdx.y, grp = blockIdx.z, groups = gridDim.z;
   const int r0 = blockIdx.x * G;
@@ -709,6 +709,7 @@ __global__ __launch_bounds__(kBlock) void k_fp_brows(const float2* __restrict__ 
 #pragma unroll
     for (int m = 0; m < VL; ++m)
       v[m] = ((rmask >> m) & 1u) ? tile[cidx[m] * TS + (g < G ? g : 0)] : make_float2(0.f, 0.f);
+    __syncthreads();                       // every row is read: the scratch reuses the tile
     PL::template run_live_tab<false>(v, scr + g * PL::LDS, t, twt, g < G);
 #pragma unroll
     for (int m = 0; m < VL; ++m) acc[m] += v[m].x * v[m].x + v[m].y * v[m].y;
@@ -1160,8 +1161,10 @@ static int run_band(const FpArgs& a, int groups, hipStream_t st) {
 
 template <class PL>
 static size_t two_pass_rows_lds(int wc) {
-  return ((size_t)wc * (groups_per_block<PL>() + 1) + (size_t)scratch_groups<PL>() * PL::LDS) *
-         sizeof(float2);
+  // the row tile and the transform scratch share the LDS (k_fp_brows)
+  const size_t tile = (size_t)wc * (groups_per_block<PL>() + 1);
+  const size_t scr = (size_t)scratch_groups<PL>() * PL::LDS;
+  return (tile > scr ? tile : scr) * sizeof(float2);
 }
 
 template <class PL>

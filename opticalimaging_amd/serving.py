@@ -96,15 +96,24 @@ class SizeClass:
 DEFAULT_MAX_BATCH = {128: 64, 256: 64, 512: 32, 1024: 16, 2048: 4}
 
 
-def size_classes(sizes, source, projector, numerics, pixel_nm=4.0, focus=0.0, max_batch=None):
+def size_classes(sizes, source, projector, numerics, pixel_nm=4.0, focus=0.0, max_batch=None,
+                 cache_dir=None):
     """One SizeClass per clip size n: kernels of the optics for an n x n clip
     of period n * pixel_nm (DecomposeTCC_SOCS.socs_kernel_set, resident on
-    the device)."""
+    the device).  With cache_dir, each set is loaded from there when a file
+    with the same optics fingerprint exists and saved there otherwise
+    (checkpoint.cached_kernel_set: a warm start skips the TCC and its
+    decomposition)."""
     from .DecomposeTCC_SOCS import socs_kernel_set
+    from .checkpoint import cached_kernel_set
     out = []
     for n in sizes:
         period = n * pixel_nm
-        ks = socs_kernel_set(source, [period, period], projector, focus, numerics, [n, n])
+        if cache_dir is not None:
+            ks, _ = cached_kernel_set(cache_dir, source, [period, period], projector, focus,
+                                      numerics, [n, n])
+        else:
+            ks = socs_kernel_set(source, [period, period], projector, focus, numerics, [n, n])
         ks.for_real_masks()          # paired set for real masks, built before serving starts
         mb = (max_batch or {}).get(n, DEFAULT_MAX_BATCH.get(n, max(1, (1 << 22) // (n * n))))
         out.append(SizeClass(n, ks, mb))

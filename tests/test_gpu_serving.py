@@ -88,3 +88,30 @@ def test_coded_requests_equal_float_requests(gpu):
     worst = max(float((ia[i] - ib[i]).abs().max()) for i in range(24))
     print("max |float - coded| over 24 requests:", worst)
     assert worst == 0.0
+
+
+def test_kernel_set_warm_start(gpu, tmp_path):
+    """serving.size_classes with a cache directory: the first call builds and
+    saves each size's kernel set, the second loads them (checkpoint) and the
+    images through the loaded sets equal those through the built ones bit
+    for bit; a changed optic misses the cache."""
+    from opticalimaging_amd import checkpoint, engine, serving, workloads
+    src, proj, num = workloads.ilt_optics()
+    sizes = [256, 512]
+    cold = serving.size_classes(sizes, src, proj, num, cache_dir=str(tmp_path))
+    files = sorted(p.name for p in tmp_path.iterdir())
+    assert len(files) == 2 and all(f.startswith("socs_") and f.endswith(".npz") for f in files)
+    warm = serving.size_classes(sizes, src, proj, num, cache_dir=str(tmp_path))
+    rng = np.random.default_rng(5)
+    for c, w in zip(cold, warm):
+        n = c.n
+        assert torch.equal(c.kernels.win, w.kernels.win) and c.kernels.band == w.kernels.band
+        m = torch.as_tensor((rng.random((2, n, n)) < 0.4).astype(np.float32)).cuda()
+        assert torch.equal(engine.socs_image(m, c.kernels), engine.socs_image(m, w.kernels))
+    ks, loaded = checkpoint.cached_kernel_set(str(tmp_path), src, [1024.0, 1024.0], proj, 0.0, num,
+                                              [256, 256])
+    assert loaded
+    proj.NA = float(proj.NA) - 0.01
+    ks2, loaded2 = checkpoint.cached_kernel_set(str(tmp_path), src, [1024.0, 1024.0], proj, 0.0, num,
+                                                [256, 256])
+    assert not loaded2 and len(list(tmp_path.iterdir())) == 3

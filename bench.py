@@ -200,7 +200,8 @@ def serving_leg(count, rate, fractions=(0.25, 0.5, 0.75, 0.9)):
     layouts, seed = request id) against device-resident K=24 kernel sets of the
     vector quasar optics, continuous batching under an 8 GB HBM budget.
 
-    Reports the saturated capacity (512 clips present at once), then p50/p99,
+    Reports the saturated capacity (512 clips present at once; median of five
+    runs), then p50/p99,
     images/s and mean batch at the offered rate `rate` and at the given
     fractions of the measured capacity (the same request stream, arrival
     times scaled)."""
@@ -230,9 +231,15 @@ def serving_leg(count, rate, fractions=(0.25, 0.5, 0.75, 0.9)):
     serving.Server(classes, ex, hbm_budget=budget, keep_images=False).run(
         [serving.Request(i, masks[i], 0.0) for i in range(nb)])
     torch.cuda.synchronize()
-    # capacity: the first 512 clips all present at t = 0, coded and as float32
-    srv = serving.Server(classes, ex, hbm_budget=budget, keep_images=False)
-    cap = serving.latency_stats(srv.run([coded(i, 0.0) for i in range(nb)]))
+    # capacity: the first 512 clips all present at t = 0, coded and as float32;
+    # the coded run is repeated and its median taken (single runs spread by
+    # ±30 % on one box, with the caching allocator still settling)
+    caps = []
+    for _ in range(5):
+        srv = serving.Server(classes, ex, hbm_budget=budget, keep_images=False)
+        caps.append(serving.latency_stats(srv.run([coded(i, 0.0) for i in range(nb)])))
+    caps.sort(key=lambda c: c["images_per_s"])
+    cap = caps[len(caps) // 2]
     srv = serving.Server(classes, ex, hbm_budget=budget, keep_images=False)
     cap_f32 = serving.latency_stats(srv.run([serving.Request(i, masks[i], 0.0) for i in range(nb)]))
 
@@ -267,6 +274,7 @@ def serving_leg(count, rate, fractions=(0.25, 0.5, 0.75, 0.9)):
            "hbm_budget_bytes": budget, "resident_bytes": int(ex.resident),
            "transfer": "uint8 codes + 2-level table per clip (1 B/pixel over the host link)",
            "capacity_images_per_s": round(cap["images_per_s"], 1),
+           "capacity_runs_images_per_s": [round(c["images_per_s"], 1) for c in caps],
            "capacity_mean_batch": round(cap["mean_batch"], 2),
            "capacity_float32_transfer_images_per_s": round(cap_f32["images_per_s"], 1),
            "load_curve": curve,

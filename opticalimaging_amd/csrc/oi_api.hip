@@ -458,7 +458,7 @@ extern "C" int oi_coherent_image(oi_plan* p, const void* spec, int batch, const 
 static bool residual_fused(const oi_plan* p) {
   const oi_geometry& g = p->g;
   return (g.path == 1 || g.path == 2) && fp_image_size_ok(g.ny) && fp_image_size_ok(g.nx) &&
-         fp_urows_blocks(g.ny, g.nx) <= g.ny;
+         fp_urows_blocks(g.ny, g.nx, 1) <= g.ny;
 }
 
 // target non-null: out = 2 (I - target), partial sums of (I - target)^2 in p->part
@@ -654,7 +654,7 @@ extern "C" int oi_socs_residual(oi_plan* p, const float* mask, int batch, const 
   hipStream_t st = (hipStream_t)stream;
   rc = coherent_impl(p, p->spec, batch, kern, nkern, resid2, target, st);
   if (rc || !loss) return rc;
-  return ilt_sum(p->part, batch * fp_urows_blocks(p->g.ny, p->g.nx), loss, st);
+  return ilt_sum(p->part, batch * fp_urows_blocks(p->g.ny, p->g.nx, batch), loss, st);
 }
 
 extern "C" int oi_spectrum_full(const void* field, int kind, int cin, const int* chans, int nchan,

@@ -28,7 +28,7 @@ struct FpArgs {
   float2* U;      // [B][qc+1][ny]
   float* out;     // [B][ny][nx]
   const float* target;  // non-null: out = 2 (I - target), part gets the sums of (I - target)^2
-  float* part;          // [B][fp_urows_blocks(ny, nx)]
+  float* part;          // [B][fp_urows_blocks(ny, nx, B)]
   StageMark mark;
   void* mark_ctx;
 };
@@ -63,7 +63,10 @@ int fp_interp(const FpArgs& a, int groups, hipStream_t st);      // marks stages
 // interpolation column and row transforms of a band already in a.Ihat
 int fp_band_image(const FpArgs& a, hipStream_t st);
 // workgroups per image of the image-row pass (partial sums of its residual form)
-int fp_urows_blocks(int ny, int nx);
+// row sets per workgroup and workgroups per image of the image-row pass for a
+// batch (its residual form writes one partial sum per workgroup)
+int fp_urows_sets(int ny, int nx, int batch);
+int fp_urows_blocks(int ny, int nx, int batch);
 // out[0] = sum of part[0 .. n) in a fixed order (oi_ilt.hip)
 int ilt_sum(const float* part, int n, float* out, hipStream_t st);
 // Two-pass coherent sum for coarse grids L in {256, 512}: B holds

@@ -819,18 +819,21 @@ __global__ __launch_bounds__(kBlock) void k_fp_ucols(const float2* __restrict__ 
 // of (I - target)^2 to part[img * gridDim.x + blockIdx.x] (the residual of a
 // least-squares fit to target, fused into the pass that produces I).
 //
-// A block images kURowSets sets of 2G rows in turn.  For Q > 0 the band tile
-// of the next set (at most Q T x 2G values) is loaded into registers while
-// the current set is transformed, so its latency hides behind the arithmetic
+// A block images rsets sets of 2G rows in turn (fp_urows_sets: up to
+// kURowSets while the grid keeps at least kURowMinBlocks workgroups, so
+// small batches keep their parallelism).  For Q > 0 the band tile of the
+// next set (at most Q T x 2G values) is loaded into registers while the
+// current set is transformed, so its latency hides behind the arithmetic
 // instead of opening every block.
 constexpr int kURowSets = 4;
+constexpr int kURowMinBlocks = 512;
 
 template <class P, int Q, bool RES>
 __global__ __launch_bounds__(kBlock) void k_fp_urows(const float2* __restrict__ U, int ny, int qc,
                                                      const float2* __restrict__ twt,
                                                      float* __restrict__ out,
                                                      const float* __restrict__ target,
-                                                     float* __restrict__ part) {
+                                                     float* __restrict__ part, int rsets) {
   constexpr int N = P::N, TT = P::T, V = P::V, G = groups_per_block<P>();
   constexpr int R = 2 * G;                    // rows per set
   constexpr int TS = R + 1;                   // tile row stride (odd: lanes reading
@@ -847,8 +850,8 @@ __global__ __launch_bounds__(kBlock) void k_fp_urows(const float2* __restrict__ 
   const int gg = live ? g : 0;
   const int img = blockIdx.y;
   const int sets = (ny + R - 1) / R;
-  const int set0 = blockIdx.x * kURowSets;
-  const int nset = min(kURowSets, sets - set0);
+  const int set0 = blockIdx.x * rsets;
+  const int nset = min(rsets, sets - set0);
   const float2* Ub = U + (size_t)img * nq * ny;
   // tile element e = q R + r of set s: U[img][q][s R + r]
   auto fetch = [&](int set, int e) -> float2 {
@@ -1084,12 +1087,13 @@ template <class P>
 static int run_urows(const FpArgs& a, hipStream_t st) {
   constexpr int G = groups_per_block<P>();
   const size_t lds = rows_lds<P>() + (size_t)(a.qc + 1) * (2 * G + 1) * sizeof(float2);
-  dim3 grid(fp_urows_blocks(a.ny, P::N), a.batch);
+  const int rsets = fp_urows_sets(a.ny, P::N, a.batch);
+  dim3 grid(fp_urows_blocks(a.ny, P::N, a.batch), a.batch);
   auto go = [&](auto kernel) -> int {
     int rc = allow(kernel, lds);
     if (rc) return rc;
     hipLaunchKernelGGL(kernel, grid, dim3(kBlock), lds, st, a.U, a.ny, a.qc, a.tw_x, a.out,
-                       a.target, a.part);
+                       a.target, a.part, rsets);
     OI_LAUNCH_CHECK("fp_urows");
     return OI_OK;
   };
@@ -1265,11 +1269,24 @@ int fp_band_image(const FpArgs& a, hipStream_t st) {
   return OI_OK;
 }
 
-int fp_urows_blocks(int ny, int nx) {
+int fp_urows_sets(int ny, int nx, int batch) {
 #define X(NN, PT)                                                                            \
   if (nx == NN) {                                                                          \
     const int sets = (ny + 2 * fp::groups_per_block<fp::PT>() - 1) / (2 * fp::groups_per_block<fp::PT>()); \
-    return (sets + fp::kURowSets - 1) / fp::kURowSets;                                     \
+    const long long r = (long long)sets * (batch > 0 ? batch : 1) / fp::kURowMinBlocks;    \
+    return r < 1 ? 1 : (r > fp::kURowSets ? fp::kURowSets : (int)r);                       \
+  }
+  OI_IMAGE_PLANS(X)
+#undef X
+  return 1;
+}
+
+int fp_urows_blocks(int ny, int nx, int batch) {
+#define X(NN, PT)                                                                            \
+  if (nx == NN) {                                                                          \
+    const int sets = (ny + 2 * fp::groups_per_block<fp::PT>() - 1) / (2 * fp::groups_per_block<fp::PT>()); \
+    const int r = fp_urows_sets(ny, nx, batch);                                            \
+    return (sets + r - 1) / r;                                                             \
   }
   OI_IMAGE_PLANS(X)
 #undef X

@@ -59,6 +59,35 @@ def test_tcc_split_bf16_error_at_m10201(gpu):
         del tcc
 
 
+@pytest.mark.parametrize("halfwidth", [4, 12, 20, 31])
+def test_tcc_split_bf16_tile_orders(gpu, halfwidth):
+    """Grids whose tile counts (M = (2h + 1)^2, ceil(M / 64) = 2, 10, 27, 63
+    tiles per side) leave partial supertiles at the edge of the triangle: every
+    tile of the split-bf16 contraction is written, the result is Hermitian
+    (exactly across tiles, where one tile is written as the other's conjugate
+    transpose; to rounding inside the diagonal tiles) and within the 3-term
+    bound of the float64 product."""
+    from opticalimaging_amd import CalculateTCCMatrix, workloads
+    src, proj, num = workloads.socs_1024_optics(order=24, grid_halfwidth=halfwidth)
+    Gs, V, f, g = O.shifted_pupil_matrix(src, [4096.0, 4096.0], proj, 0.0, num, halfwidth=halfwidth)
+    ref = float(proj.IndexImage) * sum(O.get_tcc_matrix(V, G) for G in Gs)
+    num.TCC_Precision = "split_bf16"
+    tcc, fg = CalculateTCCMatrix(src, [4096.0, 4096.0], proj, 0.0, num)
+    t = tcc.cpu().numpy()
+    M = (2 * halfwidth + 1) ** 2
+    assert t.shape == (M, M) and np.all(np.isfinite(t))
+    peak = np.max(np.abs(t))
+    assert np.max(np.abs(t - t.conj().T)) <= 1e-6 * peak
+    tb = 64                                   # the contraction's workgroup tile
+    off = np.ones((M, M), dtype=bool)
+    for i in range(0, M, tb):
+        off[i:i + tb, i:i + tb] = False
+    assert np.array_equal(t[off], t.conj().T[off])
+    err = float(np.max(np.abs(t.astype(np.complex128) - ref)) / np.max(np.abs(ref)))
+    print(f"TCC M={M} split-bf16: {err:.2e}")
+    assert err < 1e-4
+
+
 def test_bf16_tcc_through_dense_jacobi(gpu):
     """The split-bf16 TCC itself decomposed (no factor attached: the dense
     float64 Jacobi route at M = 361) gives images within 1e-4 of the float64

@@ -64,8 +64,10 @@ def log(*a):
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    # about 20 ms of timed batches after about 5 ms of warm-up: the first
+    # few dozen batches after idle run up to 10 % slower than the steady state
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=30)
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--cpu-sample", type=int, default=6,
                     help="masks in the CPU K=24 SOCS sample (a third of them for CPU Abbe)")
@@ -373,7 +375,7 @@ def ilt_leg(steps):
     theta_np, T_np = workloads.ilt_problem(batch=16, n=1024)
     theta = torch.as_tensor(theta_np).cuda()
     T = torch.as_tensor(T_np).cuda()
-    ILT.ilt(theta, T, ks, steps=2)            # warm-up
+    ILT.ilt(theta, T, ks, steps=10)           # warm-up
     torch.cuda.synchronize()
     t = time.perf_counter()
     _, losses = ILT.ilt(theta, T, ks, steps=steps)

@@ -340,6 +340,32 @@ int oi_socs_residual(oi_plan* plan, const float* mask, int batch, const void* ke
 int oi_socs_residual_adjoint(oi_plan* plan, const float* mask, int batch, const void* kern,
                              int nkern, const float* target, float* resid2, float* loss,
                              void* work, float* grad_mask, void* stream);
+/* The same gradient step without the image.  The SOCS image is band-limited,
+ * I(r) = sum_k S(k) e^{-2 pi i k.r / n} over the plan's band
+ * [-qr, qr] x [-qc, qc] (n = ny nx), so with the targets' band spectrum
+ * T~(k) = (1/n) sum_r T(r) e^{+2 pi i k.r / n} the cotangent's spectrum on
+ * that band is 2 (S - T~) and, by Parseval,
+ *   loss = n sum_band |S - T~|^2 + (sum_r T^2 - n sum_band |T~|^2);
+ * the adjoint needs nothing else.  The image's row and column transforms,
+ * the residual image and the cotangent's spectrum pass are not run.
+ *
+ * oi_ilt_target_band: tband (device, oi_ilt_target_band_bytes(plan, batch)
+ *   bytes) = the band spectrum and the per-image constant of target[batch],
+ *   once per set of targets; work is an oi_socs_adjoint work buffer (its
+ *   first region is the transform scratch).
+ * oi_socs_band_residual_adjoint: grad_mask and loss (device float, may be
+ *   null) of the step for mask[batch] against the targets tband was made
+ *   from (same batch); equal to oi_socs_residual_adjoint's to float32
+ *   rounding (the image band enters directly instead of through the rounded
+ *   image).  Plans with oi_geometry.direct set or without the residual pass
+ *   return OI_ERR_UNSUPPORTED (oi_ilt_target_band_bytes gives 0) before
+ *   anything is launched.  Calls on one plan share its scratch.             */
+size_t oi_ilt_target_band_bytes(const oi_plan* plan, int batch);
+int oi_ilt_target_band(oi_plan* plan, const float* target, int batch, void* work, void* tband,
+                       void* stream);
+int oi_socs_band_residual_adjoint(oi_plan* plan, const float* mask, int batch, const void* kern,
+                                  int nkern, const void* tband, float* loss, void* work,
+                                  float* grad_mask, void* stream);
 int oi_ilt_residual(const float* image, const float* target, float* resid2, float* loss,
                     int batch, size_t npix, float* work, void* stream);
 int oi_ilt_update(float* theta, const float* grad_m, const float* m, float lr, int batch,

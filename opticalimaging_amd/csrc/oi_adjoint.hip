@@ -581,13 +581,54 @@ static char* carve(char*& p, size_t bytes) {
   return r;
 }
 
+float2* adjoint_cotangent_spectrum(const AdjGeom& g, int batch, void* work) {
+  if (g.direct) return nullptr;
+  char* w = static_cast<char*>(work);
+  (void)carve(w, (size_t)batch * g.ny * (2 * g.qc + 1) * sizeof(float2));   // Tg
+  return (float2*)w;
+}
+
+int adjoint_band_spectrum(const AdjGeom& g, const float* img, int batch, void* work, float2* out,
+                          hipStream_t st) {
+  if (g.direct) return fail(OI_ERR_INVALID, "adjoint: no band spectrum on a direct plan");
+  const int wgc = 2 * g.qc + 1, wgr = 2 * g.qr + 1;
+  float2* Tg = static_cast<float2*>(work);     // the first region of the work buffer
+  if (fp_image_size_ok(g.ny) && fp_image_size_ok(g.nx)) {
+    // register transforms of the image sizes (same spectrum, scale and layout)
+    FpArgs a{};
+    a.field = img;
+    a.kind = OI_FIELD_REAL;
+    a.cin = 1;
+    a.nchan = 1;
+    a.chans.c[0] = 0;
+    a.batch = batch;
+    a.ny = g.ny;
+    a.nx = g.nx;
+    a.k0r = -g.qr;
+    a.wr = wgr;
+    a.k0c = -g.qc;
+    a.wc = wgc;
+    a.tw_x = g.fx.tw;
+    a.tw_y = g.fy.tw;
+    a.T = Tg;
+    a.spec = out;
+    return fp_spectrum(a, st);
+  }
+  const int chan0 = 0;
+  int rc = launch_spec_rows(img, OI_FIELD_REAL, 1, &chan0, 1, batch, g.ny, g.nx, g.fx, -g.qc, wgc,
+                            Tg, st);
+  if (rc) return rc;
+  return launch_spec_cols(Tg, batch, g.ny, g.nx, g.fy, -g.qr, wgr, wgc, out, st);
+}
+
 int launch_adjoint(const AdjGeom& g, const float* gI, const float2* spec, int batch,
                    const float2* kern, int nkern, void* work, float* grad, hipStream_t st,
                    const float2* bcols) {
+  if (!gI && g.direct) return fail(OI_ERR_INVALID, "adjoint: a direct plan needs the cotangent image");
   const size_t B = (size_t)batch, K = (size_t)nkern;
   char* w = static_cast<char*>(work);
   const int wgc = 2 * g.qc + 1, wgr = 2 * g.qr + 1;
-  float2* Tg = g.direct ? nullptr : (float2*)carve(w, B * g.ny * wgc * sizeof(float2));
+  if (!g.direct) (void)carve(w, B * g.ny * wgc * sizeof(float2));   // Tg
   float2* specg = g.direct ? nullptr : (float2*)carve(w, B * wgr * wgc * sizeof(float2));
   float2* Ecg = g.direct ? nullptr : (float2*)carve(w, B * g.lr * wgc * sizeof(float2));
   float* gamma = g.direct ? nullptr : (float*)carve(w, B * g.lr * g.lc * sizeof(float));
@@ -614,33 +655,9 @@ int launch_adjoint(const AdjGeom& g, const float* gI, const float2* spec, int ba
 
   if (!g.direct) {
     // gamma: the cotangent's spectrum on [-q, q]^2 taken to the coarse grid
-    const int chan0 = 0;
-    if (fp_image_size_ok(g.ny) && fp_image_size_ok(g.nx)) {
-      // register transforms of the image sizes (same spectrum, scale and layout)
-      FpArgs a{};
-      a.field = gI;
-      a.kind = OI_FIELD_REAL;
-      a.cin = 1;
-      a.nchan = 1;
-      a.chans.c[0] = 0;
-      a.batch = batch;
-      a.ny = g.ny;
-      a.nx = g.nx;
-      a.k0r = -g.qr;
-      a.wr = wgr;
-      a.k0c = -g.qc;
-      a.wc = wgc;
-      a.tw_x = g.fx.tw;
-      a.tw_y = g.fy.tw;
-      a.T = Tg;
-      a.spec = specg;
-      rc = fp_spectrum(a, st);
-      if (rc) return rc;
-    } else {
-      rc = launch_spec_rows(gI, OI_FIELD_REAL, 1, &chan0, 1, batch, g.ny, g.nx, g.fx, -g.qc, wgc,
-                            Tg, st);
-      if (rc) return rc;
-      rc = launch_spec_cols(Tg, batch, g.ny, g.nx, g.fy, -g.qr, wgr, wgc, specg, st);
+    // (null gI: the caller left that spectrum in the work buffer)
+    if (gI) {
+      rc = adjoint_band_spectrum(g, gI, batch, work, specg, st);
       if (rc) return rc;
     }
     if (reg) {

@@ -446,7 +446,7 @@ static int spectrum_impl(oi_plan* p, const void* field, int kind, int cin, const
 }
 
 static int coherent_impl(oi_plan* p, const void* spec, int batch, const void* kern, int nkern,
-                         float* out, const float* target, hipStream_t st);
+                         float* out, const float* target, hipStream_t st, bool band_only = false);
 
 extern "C" int oi_coherent_image(oi_plan* p, const void* spec, int batch, const void* kern,
                                  int nkern, float* out, void* stream) {
@@ -461,11 +461,13 @@ static bool residual_fused(const oi_plan* p) {
          fp_urows_blocks(g.ny, g.nx, 1) <= g.ny;
 }
 
-// target non-null: out = 2 (I - target), partial sums of (I - target)^2 in p->part
+// target non-null: out = 2 (I - target), partial sums of (I - target)^2 in p->part.
+// band_only (plans with the residual pass): the band p->Ihat of the image, no image.
 static int coherent_impl(oi_plan* p, const void* spec, int batch, const void* kern, int nkern,
-                         float* out, const float* target, hipStream_t st) {
-  if (!p || !spec || !kern || !out) return fail(OI_ERR_INVALID, "coherent_image: null argument");
-  if (target && !residual_fused(p))
+                         float* out, const float* target, hipStream_t st, bool band_only) {
+  if (!p || !spec || !kern || (!out && !band_only))
+    return fail(OI_ERR_INVALID, "coherent_image: null argument");
+  if ((target || band_only) && !residual_fused(p))
     return fail(OI_ERR_UNSUPPORTED, "socs_residual: plan without the compile-time-size image pass");
   if (batch < 1 || batch > p->g.max_batch) return fail(OI_ERR_INVALID, "batch outside [1, max_batch]");
   if (nkern < 1) return fail(OI_ERR_INVALID, "coherent_image: nkern must be >= 1");
@@ -498,6 +500,7 @@ static int coherent_impl(oi_plan* p, const void* spec, int batch, const void* ke
     c.R = p->R;
     int rc = fp_two_pass(a, c, p->Bbuf, fast_io ? 0 : 1, st);
     if (rc) return rc;
+    if (band_only) return fp_band(a, c.groups, st);
     mark(p, 3, st);
     if (fast_io) {
       rc = fp_interp(a, c.groups, st);
@@ -541,6 +544,7 @@ static int coherent_impl(oi_plan* p, const void* spec, int batch, const void* ke
     c.tw_l = p->flr.tw;
     c.tw_m = p->flr.tw;
     c.R = p->R;
+    if (band_only) return fp_image_band(a, c, st);
     int rc = fp_image(a, c, st);
     mark_done(p);
     return rc;
@@ -722,6 +726,83 @@ extern "C" int oi_socs_residual_adjoint(oi_plan* p, const float* mask, int batch
   const float2* bcols = p->g.path == 2 ? p->Bbuf : nullptr;
   return launch_adjoint(adj_geom(p), resid2, p->spec, batch, static_cast<const float2*>(kern), nkern,
                         work, grad_mask, (hipStream_t)stream, bcols);
+}
+
+// The band form (plans with the residual pass, not direct ones): the image's
+// band and the targets' band spectrum give the cotangent spectrum and the loss
+// without the image (oi_ilt.hip states the identities).
+static bool band_residual_ok(const oi_plan* p) {
+  // the loss's partial sums go to p->part (ny per image)
+  return p->g.nchan == 1 && !p->g.direct && residual_fused(p) &&
+         ilt_band_blocks(p->g.qr, p->g.qc) <= p->g.ny;
+}
+
+static size_t band_values(const oi_plan* p) {
+  return (size_t)(2 * p->g.qr + 1) * (2 * p->g.qc + 1);
+}
+
+// the target band buffer: c64 [batch][2 qr + 1][2 qc + 1], then float [batch]
+// (the constants) at the next 256-byte boundary
+static size_t band_const_offset(const oi_plan* p, int batch) {
+  return ((size_t)batch * band_values(p) * sizeof(float2) + 255) & ~(size_t)255;
+}
+
+extern "C" size_t oi_ilt_target_band_bytes(const oi_plan* p, int batch) {
+  if (!p || batch < 1 || !band_residual_ok(p)) return 0;
+  const size_t B = (size_t)batch;
+  return band_const_offset(p, batch) + B * sizeof(float);
+}
+
+extern "C" int oi_ilt_target_band(oi_plan* p, const float* target, int batch, void* work,
+                                  void* tband, void* stream) {
+  if (!p || !target || !work || !tband) return fail(OI_ERR_INVALID, "ilt_target_band: null argument");
+  if (!band_residual_ok(p))
+    return fail(OI_ERR_UNSUPPORTED, "ilt_target_band: plan without the band form of the residual");
+  if (batch < 1 || batch > p->g.max_batch) return fail(OI_ERR_INVALID, "batch outside [1, max_batch]");
+  hipStream_t st = (hipStream_t)stream;
+  const AdjGeom ag = adj_geom(p);
+  float2* band = static_cast<float2*>(tband);
+  int rc = adjoint_band_spectrum(ag, target, batch, work, band, st);
+  if (rc) return rc;
+  float* tconst = reinterpret_cast<float*>(static_cast<char*>(tband) + band_const_offset(p, batch));
+  // the band spectrum is done with the work buffer's transform scratch (same
+  // stream), which then holds the constants' partial sums
+  return ilt_target_const(target, (size_t)p->g.ny * p->g.nx, band, band_values(p), batch, work,
+                          tconst, st);
+}
+
+extern "C" int oi_socs_band_residual_adjoint(oi_plan* p, const float* mask, int batch,
+                                             const void* kern, int nkern, const void* tband,
+                                             float* loss, void* work, float* grad_mask,
+                                             void* stream) {
+  if (!p || !mask || !kern || !tband || !work || !grad_mask)
+    return fail(OI_ERR_INVALID, "socs_band_residual_adjoint: null argument");
+  if (!band_residual_ok(p))
+    return fail(OI_ERR_UNSUPPORTED, "socs_band_residual_adjoint: plan without the band form of the residual");
+  if (batch < 1 || batch > p->g.max_batch) return fail(OI_ERR_INVALID, "batch outside [1, max_batch]");
+  if (nkern < 1 || nkern > p->g.max_kernels)
+    return fail(OI_ERR_INVALID, "socs_band_residual_adjoint: nkern outside [1, max_kernels]");
+  hipStream_t st = (hipStream_t)stream;
+  const int chan0 = 0;
+  const int slots = p->slots;
+  p->slots = 0;  // stage timing covers the imaging calls only
+  int rc = spectrum_impl(p, mask, OI_FIELD_REAL, 1, &chan0, batch, p->spec, st);
+  if (!rc) rc = coherent_impl(p, p->spec, batch, kern, nkern, nullptr, nullptr, st, true);
+  p->slots = slots;
+  if (rc) return rc;
+  const AdjGeom ag = adj_geom(p);
+  const float2* band = static_cast<const float2*>(tband);
+  const float* tconst = reinterpret_cast<const float*>(static_cast<const char*>(tband) +
+                                                       band_const_offset(p, batch));
+  rc = ilt_band_cotangent(p->Ihat, p->g.qr, p->g.qc, band, tconst,
+                          (float)p->g.ny * (float)p->g.nx, batch,
+                          adjoint_cotangent_spectrum(ag, batch, work), p->part, st);
+  if (rc) return rc;
+  if (loss && (rc = ilt_sum(p->part, batch * ilt_band_blocks(p->g.qr, p->g.qc), loss, st)))
+    return rc;
+  const float2* bcols = p->g.path == 2 ? p->Bbuf : nullptr;
+  return launch_adjoint(ag, nullptr, p->spec, batch, static_cast<const float2*>(kern), nkern, work,
+                        grad_mask, st, bcols);
 }
 
 extern "C" size_t oi_socs_adjoint_work_bytes(const oi_plan* p, int batch, int nkern) {

@@ -60,15 +60,32 @@ bool fp_one_pass_ok(int L);             // the one-pass coherent kernel serves L
 int fp_spectrum(const FpArgs& a, hipStream_t st);  // marks stage 1 between its launches
 int fp_image(const FpArgs& a, const FpCoh& c, hipStream_t st);  // marks stages 3, 4, 5
 int fp_interp(const FpArgs& a, int groups, hipStream_t st);      // marks stages 4, 5
+// the one-pass coherent sum and the band Ihat of the image, without the image
+int fp_image_band(const FpArgs& a, const FpCoh& c, hipStream_t st);
+// the band Ihat from partial sums R (the first kernel of fp_interp)
+int fp_band(const FpArgs& a, int groups, hipStream_t st);
 // interpolation column and row transforms of a band already in a.Ihat
 int fp_band_image(const FpArgs& a, hipStream_t st);
-// workgroups per image of the image-row pass (partial sums of its residual form)
 // row sets per workgroup and workgroups per image of the image-row pass for a
 // batch (its residual form writes one partial sum per workgroup)
 int fp_urows_sets(int ny, int nx, int batch);
 int fp_urows_blocks(int ny, int nx, int batch);
 // out[0] = sum of part[0 .. n) in a fixed order (oi_ilt.hip)
 int ilt_sum(const float* part, int n, float* out, hipStream_t st);
+// Band form of the least-squares loss (oi_ilt.hip).  Ihat [batch][qc + 1][2 qr + 1]
+// is the image band (the layout k_fp_band writes); tband [batch][2 qr + 1][2 qc + 1]
+// the targets' band spectrum and tconst[b] = sum T^2 - n sum_band |T~|^2.
+// Writes the cotangent spectrum 2 (S - T~) to specg (tband's layout) and
+// ilt_band_blocks(qr, qc) partial sums of the loss per image to part.
+int ilt_band_blocks(int qr, int qc);
+int ilt_band_cotangent(const float2* Ihat, int qr, int qc, const float2* tband,
+                       const float* tconst, float n, int batch, float2* specg, float* part,
+                       hipStream_t st);
+// tconst[b] = sum_j T[b][j]^2 - n sum_band |tband[b]|^2 (accumulated in
+// float64); scratch: ilt_target_const_scratch(batch) bytes of device memory
+size_t ilt_target_const_scratch(int batch);
+int ilt_target_const(const float* target, size_t npix, const float2* tband, size_t nband,
+                     int batch, void* scratch, float* tconst, hipStream_t st);
 // Two-pass coherent sum for coarse grids L in {256, 512}: B holds
 // batch * nkern * nchan * wc * L c64; R in [img][q][t] (generic_layout = 0) or
 // [img][t][q] (1).

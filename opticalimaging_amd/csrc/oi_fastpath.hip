@@ -1229,7 +1229,7 @@ int fp_two_pass(const FpArgs& a, const FpCoh& c, float2* B, int generic_layout, 
 
 // Band extraction + column and row transforms to the image (stages 4, 5) for
 // partial sums R[img][groups][q][t] on an L x L coarse grid.
-int fp_interp(const FpArgs& a, int groups, hipStream_t st) {
+int fp_band(const FpArgs& a, int groups, hipStream_t st) {
   int rc = OI_ERR_UNSUPPORTED;
 #define X(NN, A, B, NT) if (a.L == NN) rc = fp::run_band<reg::GFFT<NN, A, B>>(a, groups, st);
   OI_COARSE_PLANS(X)
@@ -1239,6 +1239,12 @@ int fp_interp(const FpArgs& a, int groups, hipStream_t st) {
   OI_TWO_PASS_PLANS(X)
 #undef X
   if (rc) return rc == OI_ERR_UNSUPPORTED ? fail(rc, "fast path: unsupported coarse size") : rc;
+  return OI_OK;
+}
+
+int fp_interp(const FpArgs& a, int groups, hipStream_t st) {
+  int rc = fp_band(a, groups, st);
+  if (rc) return rc;
   rc = OI_ERR_UNSUPPORTED;
 #define X(NN, PT) if (a.ny == NN) rc = fp::run_ucols<fp::PT>(a, st);
   OI_IMAGE_PLANS(X)
@@ -1388,14 +1394,26 @@ int fp_spectrum(const FpArgs& a, hipStream_t st) {
   return OI_OK;
 }
 
-int fp_image(const FpArgs& a, const FpCoh& c, hipStream_t st) {
+static int fp_coherent(const FpArgs& a, const FpCoh& c, hipStream_t st) {
   int rc = OI_ERR_UNSUPPORTED;
 #define X(NN, A, B, NT) if (a.L == NN) rc = fp::run_coherent<reg::GFFT<NN, A, B>, NT>(a, c, st);
   OI_COARSE_PLANS(X)
 #undef X
   if (rc) return rc == OI_ERR_UNSUPPORTED ? fail(rc, "fast path: unsupported coarse size") : rc;
+  return OI_OK;
+}
+
+int fp_image(const FpArgs& a, const FpCoh& c, hipStream_t st) {
+  int rc = fp_coherent(a, c, st);
+  if (rc) return rc;
   if (a.mark) a.mark(a.mark_ctx, 3);
   return fp_interp(a, c.groups, st);
+}
+
+int fp_image_band(const FpArgs& a, const FpCoh& c, hipStream_t st) {
+  int rc = fp_coherent(a, c, st);
+  if (rc) return rc;
+  return fp_band(a, c.groups, st);
 }
 
 }  // namespace oi

@@ -2,10 +2,12 @@
 //   oi_ilt_residual : d = 2 (I - T) and the loss sum (I - T)^2
 //   oi_ilt_update   : g = dL/dm m (1 - m); theta -= lr g / rms(g) per image;
 //                     optionally m = sigmoid(theta) for the next step
+// and the band form of the residual (the cotangent spectrum and the loss from
+// the image's band, for oi_socs_band_residual_adjoint).
 // Every reduction runs over a fixed split (kIltBlocks workgroups per image,
 // then the partial sums in a fixed order), so results do not depend on the
-// batch or on timing.  These passes stream 4-byte values: HBM-bound, float4
-// loads and stores when the pointers and the image size allow.
+// batch or on timing.  The elementwise passes stream 4-byte values:
+// HBM-bound, float4 loads and stores when the pointers and the image size allow.
 #include "oi_common.h"
 #include "oi_fastpath.h"
 #include "opticalimaging_amd.h"
@@ -163,6 +165,123 @@ __global__ __launch_bounds__(kIltThreads) void k_ilt_update(float* theta, const 
 
 bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; }
 
+// Band form of the least-squares loss L = sum_r (I - T)^2 for band-limited
+// images I(r) = sum_k S(k) e^{-2 pi i k.r / n} (n = ny nx, k in the band
+// [-qr, qr] x [-qc, qc]) and T~(k) = (1/n) sum_r T(r) e^{+2 pi i k.r / n}:
+//   (1/n) sum_r 2 (I - T)(r) e^{+2 pi i k.r / n} = 2 (S(k) - T~(k))  (k in the band)
+//   L = n sum_band |S - T~|^2 + (sum_r T^2 - n sum_band |T~|^2)     (Parseval)
+// S comes from the band Ihat[q][qr + qrm] the image synthesis reads (q >= 0;
+// q < 0 is the conjugate of (-qr, -q)); the q = 0 row enters through its
+// Hermitian part, which is the part a real image carries.
+// The band Ihat is [q][i] (i = qr + qrm fastest) and the cotangent spectrum
+// [i][j] (j = kc + qc fastest): each workgroup moves a 32 x 32 tile through
+// LDS so that both are read and written along their rows.
+constexpr int kBandTile = 32;
+
+__device__ __forceinline__ float2 band_value(const float2* __restrict__ Ib, int nb, int qr, int i,
+                                             int kc) {
+  if (kc > 0) return Ib[(size_t)kc * nb + i];
+  const int im = 2 * qr - i;                 // the order (-kr)
+  if (kc < 0) {
+    const float2 u = Ib[(size_t)(-kc) * nb + im];
+    return make_float2(u.x, -u.y);
+  }
+  const float2 a = Ib[i], b = Ib[im];
+  return make_float2(0.5f * (a.x + b.x), 0.5f * (a.y - b.y));
+}
+
+__global__ __launch_bounds__(kIltThreads) void k_ilt_band_cotangent(
+    const float2* __restrict__ Ihat, int qr, int qc, const float2* __restrict__ tband,
+    const float* __restrict__ tconst, float n, float2* __restrict__ specg,
+    float* __restrict__ part) {
+  constexpr int TT = kBandTile, ROWS = kIltThreads / TT;
+  __shared__ float2 tile[TT][TT + 1];        // [jj][ii]
+  __shared__ float red[kIltThreads / 64];
+  const int img = blockIdx.z;
+  const int nb = 2 * qr + 1, wgc = 2 * qc + 1;
+  const int i0 = blockIdx.y * TT, j0 = blockIdx.x * TT;
+  const float2* Ib = Ihat + (size_t)img * (qc + 1) * nb;
+  const int lane = threadIdx.x % TT, row = threadIdx.x / TT;
+  // S along i (the band's rows)
+#pragma unroll
+  for (int r = row; r < TT; r += ROWS) {
+    const int i = i0 + lane, j = j0 + r;
+    tile[r][lane] = (i < nb && j < wgc) ? band_value(Ib, nb, qr, i, j - qc) : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  // 2 (S - T~) along j (the cotangent spectrum's rows)
+  const size_t E = (size_t)nb * wgc;
+  const float2* Tb = tband + (size_t)img * E;
+  float2* Gb = specg + (size_t)img * E;
+  float acc = 0.f;
+#pragma unroll
+  for (int r = row; r < TT; r += ROWS) {
+    const int i = i0 + r, j = j0 + lane;
+    if (i < nb && j < wgc) {
+      const float2 s = tile[lane][r];
+      const float2 t = Tb[(size_t)i * wgc + j];
+      const float dx = s.x - t.x, dy = s.y - t.y;
+      acc += dx * dx + dy * dy;
+      Gb[(size_t)i * wgc + j] = make_float2(2.f * dx, 2.f * dy);
+    }
+  }
+  const float sum = block_sum(acc, red);
+  if (threadIdx.x == 0) {
+    const int blk = blockIdx.y * gridDim.x + blockIdx.x;
+    part[(size_t)img * gridDim.x * gridDim.y + blk] = n * sum + (blk == 0 ? tconst[img] : 0.f);
+  }
+}
+
+// Per-image constant sum T^2 - n sum_band |T~|^2 in float64, in two fixed
+// splits (kConstBlocks partial sums of T^2 per image, then one workgroup per
+// image), so the result does not depend on timing.
+constexpr int kConstBlocks = 64;
+
+__global__ __launch_bounds__(kIltThreads) void k_ilt_target_sq(const float* __restrict__ target,
+                                                               size_t npix,
+                                                               double* __restrict__ part) {
+  __shared__ double red[kIltThreads / 64];
+  const float* T = target + (size_t)blockIdx.y * npix;
+  double a = 0.0;
+  for (size_t i = (size_t)blockIdx.x * kIltThreads + threadIdx.x; i < npix;
+       i += (size_t)kConstBlocks * kIltThreads) {
+    const double v = T[i];
+    a += v * v;
+  }
+  a = wave_sum(a);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = a;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+#pragma unroll
+    for (int w = 0; w < kIltThreads / 64; ++w) s += red[w];
+    part[(size_t)blockIdx.y * kConstBlocks + blockIdx.x] = s;
+  }
+}
+
+__global__ __launch_bounds__(kIltThreads) void k_ilt_target_const(
+    const double* __restrict__ sq, const float2* __restrict__ tband, size_t nband, double n,
+    float* __restrict__ tconst) {
+  __shared__ double red[kIltThreads / 64];
+  const int img = blockIdx.x;
+  const float2* Tb = tband + (size_t)img * nband;
+  double b = 0.0;
+  for (size_t i = threadIdx.x; i < nband; i += kIltThreads) {
+    const float2 u = Tb[i];
+    b += (double)u.x * u.x + (double)u.y * u.y;
+  }
+  b = wave_sum(b);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = b;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0.0, s = 0.0;
+    for (int k = 0; k < kConstBlocks; ++k) a += sq[(size_t)img * kConstBlocks + k];
+#pragma unroll
+    for (int w = 0; w < kIltThreads / 64; ++w) s += red[w];
+    tconst[img] = (float)(a - n * s);
+  }
+}
+
 }  // namespace
 }  // namespace oi
 
@@ -170,6 +289,38 @@ namespace oi {
 int ilt_sum(const float* part, int n, float* out, hipStream_t st) {
   hipLaunchKernelGGL(k_ilt_total, dim3(1), dim3(64), 0, st, part, n, out);
   OI_LAUNCH_CHECK("ilt_total");
+  return OI_OK;
+}
+
+int ilt_band_blocks(int qr, int qc) {
+  const int t = kBandTile;
+  return ((2 * qr + 1 + t - 1) / t) * ((2 * qc + 1 + t - 1) / t);
+}
+
+int ilt_band_cotangent(const float2* Ihat, int qr, int qc, const float2* tband,
+                       const float* tconst, float n, int batch, float2* specg, float* part,
+                       hipStream_t st) {
+  const int t = kBandTile;
+  const dim3 grid((2 * qc + 1 + t - 1) / t, (2 * qr + 1 + t - 1) / t, batch);
+  hipLaunchKernelGGL(k_ilt_band_cotangent, grid, dim3(kIltThreads), 0, st, Ihat, qr, qc, tband,
+                     tconst, n, specg, part);
+  OI_LAUNCH_CHECK("ilt_band_cotangent");
+  return OI_OK;
+}
+
+size_t ilt_target_const_scratch(int batch) {
+  return (size_t)batch * kConstBlocks * sizeof(double);
+}
+
+int ilt_target_const(const float* target, size_t npix, const float2* tband, size_t nband,
+                     int batch, void* scratch, float* tconst, hipStream_t st) {
+  double* sq = static_cast<double*>(scratch);
+  hipLaunchKernelGGL(k_ilt_target_sq, dim3(kConstBlocks, batch), dim3(kIltThreads), 0, st, target,
+                     npix, sq);
+  OI_LAUNCH_CHECK("ilt_target_sq");
+  hipLaunchKernelGGL(k_ilt_target_const, dim3(batch), dim3(kIltThreads), 0, st, sq, tband, nband,
+                     (double)npix, tconst);
+  OI_LAUNCH_CHECK("ilt_target_const");
   return OI_OK;
 }
 }  // namespace oi

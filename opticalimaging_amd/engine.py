@@ -554,6 +554,76 @@ def socs_residual_adjoint(masks: torch.Tensor, ks: KernelSet, target: torch.Tens
     return True
 
 
+def ilt_target_band(target: torch.Tensor, ks: KernelSet, work: torch.Tensor,
+                    plan: Plan | None = None) -> torch.Tensor | None:
+    """The targets' band spectrum and per-image constants for
+    socs_band_residual_adjoint (oi_ilt_target_band), as a uint8 device
+    tensor; None, with nothing computed, when the plan has no band form of
+    the residual (direct plans, plans without the residual pass).  work: an
+    adjoint_work buffer of the same plan and batch."""
+    B = int(target.shape[0]) if isinstance(target, torch.Tensor) and target.dim() == 3 else -1
+    if B < 1:
+        raise ValueError("target must be a (B, ny, nx) device tensor with B >= 1")
+    _device_tensor("target", target, torch.float32, (B, ks.ny, ks.nx))
+    ks = _real_set(ks, True)
+    if plan is None:
+        plan = plan_for(ks, 1, B)
+    _check_plan(plan, ks, 1, B)
+    lib = nat.load()
+    n = int(lib.oi_ilt_target_band_bytes(plan.handle, B))
+    if n == 0:
+        return None
+    need = int(lib.oi_socs_adjoint_work_bytes(plan.handle, B, ks.K))
+    if not (isinstance(work, torch.Tensor) and work.is_cuda and work.dtype == torch.uint8
+            and work.is_contiguous() and work.numel() >= need):
+        raise ValueError(f"work must be a contiguous uint8 device tensor of at least {need} bytes")
+    tband = torch.empty(n, dtype=torch.uint8, device=target.device)
+    nat.check(lib.oi_ilt_target_band(plan.handle, nat.ptr(target), B, nat.ptr(work), nat.ptr(tband),
+                                     nat.stream()))
+    return tband
+
+
+def socs_band_residual_adjoint(masks: torch.Tensor, ks: KernelSet, tband: torch.Tensor,
+                               grad: torch.Tensor, work: torch.Tensor,
+                               loss: torch.Tensor | None = None, plan: Plan | None = None) -> None:
+    """grad = the mask gradient of sum (I - T)^2 and loss[0] = that sum, for
+    real masks [B, ny, nx] against the targets `tband` was made from
+    (ilt_target_band, same plan and batch), from the image's band without
+    forming the image (oi_socs_band_residual_adjoint; equal to
+    socs_residual_adjoint's gradient and loss to float32 rounding)."""
+    if not isinstance(masks, torch.Tensor) or masks.dim() != 3:
+        raise ValueError("masks must be a (B, ny, nx) device tensor")
+    B = int(masks.shape[0])
+    shape = (B, ks.ny, ks.nx)
+    for name, t in (("masks", masks), ("grad", grad)):
+        _device_tensor(name, t, torch.float32, shape)
+    if loss is not None and not (loss.is_cuda and loss.dtype == torch.float32 and loss.numel() >= 1
+                                 and loss.is_contiguous()):
+        raise ValueError("loss must be a contiguous float32 device tensor with one element")
+    if B == 0:
+        if loss is not None:
+            loss.zero_()
+        return
+    ks = _real_set(ks, True)
+    if plan is None:
+        plan = plan_for(ks, 1, B)
+    _check_plan(plan, ks, 1, B)
+    lib = nat.load()
+    nb = int(lib.oi_ilt_target_band_bytes(plan.handle, B))
+    if nb == 0:
+        raise ValueError("the plan has no band form of the residual")
+    if not (isinstance(tband, torch.Tensor) and tband.is_cuda and tband.dtype == torch.uint8
+            and tband.is_contiguous() and tband.numel() >= nb):
+        raise ValueError(f"tband must be an ilt_target_band tensor of {nb} bytes")
+    need = int(lib.oi_socs_adjoint_work_bytes(plan.handle, B, ks.K))
+    if not (isinstance(work, torch.Tensor) and work.is_cuda and work.dtype == torch.uint8
+            and work.is_contiguous() and work.numel() >= need):
+        raise ValueError(f"work must be a contiguous uint8 device tensor of at least {need} bytes")
+    nat.check(lib.oi_socs_band_residual_adjoint(plan.handle, nat.ptr(masks), B, nat.ptr(ks.win),
+                                                ks.K, nat.ptr(tband), nat.ptr(loss), nat.ptr(work),
+                                                nat.ptr(grad), nat.stream()))
+
+
 def adjoint_work(ks: KernelSet, batch: int, plan: Plan | None = None) -> torch.Tensor:
     """Device scratch for socs_adjoint / socs_residual_adjoint on `plan`."""
     ks = _real_set(ks, True)

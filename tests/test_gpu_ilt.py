@@ -201,6 +201,70 @@ def test_socs_residual_adjoint_equals_separate_calls(gpu, case):
     assert peak > 0 and err <= 1e-6
 
 
+@pytest.mark.parametrize("case", ["one-pass 256", "two-pass 1024"])
+def test_band_residual_adjoint_matches_image_form(gpu, case):
+    """oi_socs_band_residual_adjoint (cotangent spectrum 2 (S - T~) from the
+    image band S and the targets' band T~; loss by Parseval plus the
+    per-target constant) against oi_socs_residual_adjoint (image formed,
+    2 (I - T) transformed): the loss to 1e-5 relative and the mask gradient
+    to 2e-6 of its peak (the same quantities with the image's rounding left
+    out).  Two calls in a row on one plan, the second with other masks."""
+    from opticalimaging_amd import engine, workloads
+    from opticalimaging_amd.DecomposeTCC_SOCS import socs_kernel_set
+    if case == "one-pass 256":
+        n, period = 256, 1024.0
+        src, proj, num = annular(), projector(), numerics(order=24)
+        tcc, fg = O.tcc_matrix(src, [period, period], proj, 0.0, num)
+        kern = O.decompose_tcc_socs(tcc, fg, [n, n], num)
+        ks = engine.KernelSet.from_dense(torch.as_tensor(kern.astype(np.complex64)))
+    else:
+        n = 1024
+        src, proj, num = workloads.ilt_optics()
+        ks = socs_kernel_set(src, [4.0 * n, 4.0 * n], proj, 0.0, num, [n, n])
+    B = 3
+    rng = np.random.default_rng(n + 2)
+    T = torch.as_tensor((rng.random((B, n, n)) < 0.3).astype(np.float32)).cuda()
+    plan = engine.plan_for(ks, 1, B)
+    work = engine.adjoint_work(ks, B, plan)
+    tband = engine.ilt_target_band(T, ks, work, plan)
+    assert tband is not None
+    for trial in range(2):
+        m = torch.as_tensor(rng.random((B, n, n)).astype(np.float32)).cuda()
+        d1, g1 = torch.empty_like(m), torch.empty_like(m)
+        l1 = torch.zeros(1, device="cuda")
+        assert engine.socs_residual_adjoint(m, ks, T, d1, g1, work, l1, plan=plan)
+        g2 = torch.full_like(m, 5.0)
+        l2 = torch.full((1,), -1.0, device="cuda")
+        engine.socs_band_residual_adjoint(m, ks, tband, g2, work, l2, plan=plan)
+        torch.cuda.synchronize()
+        lerr = abs(float(l2) - float(l1)) / float(l1)
+        peak = float(g1.abs().max())
+        gerr = float((g1 - g2).abs().max()) / peak
+        print(f"{case} call {trial}: loss {float(l2):.6e} vs {float(l1):.6e} ({lerr:.1e}), "
+              f"gradient {gerr:.2e} of its peak")
+        assert lerr < 1e-5 and peak > 0 and gerr <= 2e-6
+
+
+def test_band_residual_needs_the_band_form(gpu):
+    """Plans without the fused image pass have no band form: the byte count
+    is 0, ilt_target_band returns None and the step raises."""
+    from opticalimaging_amd import engine
+    import opticalimaging_amd._native as nat
+    rng = np.random.default_rng(5)
+    win = torch.as_tensor((rng.standard_normal((3, 9, 11)) + 1j * rng.standard_normal((3, 9, 11)))
+                          .astype(np.complex64)).cuda()
+    ks = engine.KernelSet(win, -4, -5, 96, 160)
+    plan = engine.plan_for(ks, 1, 2)
+    assert plan.info()["path"] == 0
+    assert nat.load().oi_ilt_target_band_bytes(plan.handle, 2) == 0
+    T = torch.zeros((2, 96, 160), device="cuda")
+    work = engine.adjoint_work(ks, 2, plan)
+    assert engine.ilt_target_band(T, ks, work, plan) is None
+    with pytest.raises(ValueError):
+        engine.socs_band_residual_adjoint(T, ks, torch.zeros(16, dtype=torch.uint8, device="cuda"),
+                                          torch.empty_like(T), work, plan=plan)
+
+
 def test_socs_residual_reports_generic_plans(gpu):
     """A plan on the generic path (96 x 160) has no fused residual: the call
     returns False and leaves the outputs untouched."""

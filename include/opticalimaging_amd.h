@@ -366,6 +366,19 @@ int oi_ilt_target_band(oi_plan* plan, const float* target, int batch, void* work
 int oi_socs_band_residual_adjoint(oi_plan* plan, const float* mask, int batch, const void* kern,
                                   int nkern, const void* tband, float* loss, void* work,
                                   float* grad_mask, void* stream);
+/* oi_ilt_band_step: one whole normalised gradient step on the logits,
+ *   m = sigmoid(theta) [batch][ny][nx] (device, updated in place): the loss
+ *   (device float, may be null) and dL/dm as oi_socs_band_residual_adjoint,
+ *   grad_theta = dL/dm m (1 - m) (device [batch][ny][nx], written), then per
+ *   image theta -= lr grad_theta / (sqrt(sum grad_theta^2 / npix) + 1e-30),
+ *   as oi_ilt_update.  m is never stored: the spectrum pass and the
+ *   gradient's last pass take the sigmoid of theta as they load it, and the
+ *   norm's partial sums come from the pass that writes grad_theta.  Equal to
+ *   oi_socs_band_residual_adjoint followed by oi_ilt_update to float32
+ *   rounding.  OI_ERR_UNSUPPORTED as oi_socs_band_residual_adjoint.        */
+int oi_ilt_band_step(oi_plan* plan, float* theta, int batch, const void* kern, int nkern,
+                     const void* tband, float lr, float* loss, void* work, float* grad_theta,
+                     void* stream);
 int oi_ilt_residual(const float* image, const float* target, float* resid2, float* loss,
                     int batch, size_t npix, float* work, void* stream);
 int oi_ilt_update(float* theta, const float* grad_m, const float* m, float lr, int batch,

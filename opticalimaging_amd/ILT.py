@@ -70,16 +70,17 @@ def ilt(theta: torch.Tensor, target: torch.Tensor, kernels, steps: int = 50, lr:
     lr times its gradient divided by the gradient's RMS); returns (theta,
     per-step losses tensor on the device).
 
-    The gradient is the one ``ilt_loss`` gives through autograd, written out
-    so that each step is two library calls: the loss and dL/dm = adjoint(m,
-    2 (I - T)) with I = SOCS(m) in one (oi_socs_band_residual_adjoint: the
-    image is band-limited, so the cotangent's spectrum is 2 (band of I - band
-    of T) and the loss follows by Parseval from the band and a per-target
-    constant, both computed once before the loop; the image itself is not
-    formed.  Direct plans use oi_socs_residual_adjoint, and plans without the
-    fused image pass oi_socs_image, oi_ilt_residual and oi_socs_adjoint);
-    then dL/dtheta = dL/dm m (1 - m), its per-image RMS, the update of theta
-    and the next m = sigmoid(theta) (oi_ilt_update).  The loop stays on the
+    The gradient is the one ``ilt_loss`` gives through autograd, written out.
+    Where the plan has the band form, each step is one library call
+    (oi_ilt_band_step): the image is band-limited, so the spectrum of the
+    cotangent 2 (I - T) is 2 (band of I - band of T) and the loss follows by
+    Parseval from the band and a per-target constant (the targets' band and
+    constant are computed once, oi_ilt_target_band); the image is not formed,
+    and m = sigmoid(theta) is taken as theta is loaded, never stored.  The
+    call then applies dL/dtheta = dL/dm m (1 - m), its per-image RMS and the
+    update of theta.  Elsewhere a step is oi_socs_residual_adjoint (direct
+    plans) or oi_socs_image, oi_ilt_residual and oi_socs_adjoint (plans
+    without the fused image pass), then oi_ilt_update.  The loop stays on the
     device (no host synchronisation per step)."""
     ks = kernel_set(kernels)
     theta = theta.detach().to(torch.float32).contiguous().clone()
@@ -95,35 +96,34 @@ def ilt(theta: torch.Tensor, target: torch.Tensor, kernels, steps: int = 50, lr:
     npix = int(theta[0].numel())
     plan = engine.plan_for(ks, 1, B)
     lib = nat.load()
-    work = torch.empty(lib.oi_ilt_work_floats(B), dtype=torch.float32, device=theta.device)
-    img = None                     # the image itself, only where the residual is not fused
-    d = None                       # 2 (I - T), only where the band form is not available
     g = torch.empty_like(theta)
-    m = torch.sigmoid(theta)
-    st = nat.stream()
     awork = engine.adjoint_work(ks, B, plan)
     tband = engine.ilt_target_band(target, ks, awork, plan)
+    if tband is not None:
+        # the whole step from theta, the image band and the targets' band
+        for i in range(steps):
+            engine.ilt_band_step(theta, ks, tband, lr, g, awork, losses[i:i + 1], plan=plan)
+        return theta, losses
+    # d = 2 (I - T), the loss and g = dL/dm: in the pass that produces I, with
+    # the mask spectrum shared with the adjoint, where the plan has that pass;
+    # otherwise the image, the residual pass and the adjoint one after another
+    work = torch.empty(lib.oi_ilt_work_floats(B), dtype=torch.float32, device=theta.device)
+    img = None                     # the image itself, only where the residual is not fused
+    d = torch.empty_like(theta)
+    m = torch.sigmoid(theta)
+    st = nat.stream()
     fused = True
     for i in range(steps):
-        # the loss and g = dL/dm from the image band where the plan has the
-        # band form; else d = 2 (I - T) in the pass that produces I with the
-        # mask spectrum shared with the adjoint; else the image, the residual
-        # pass and the adjoint one after another
-        if tband is not None:
-            engine.socs_band_residual_adjoint(m, ks, tband, g, awork, losses[i:i + 1], plan=plan)
-        else:
-            if d is None:
-                d = torch.empty_like(theta)
-            if fused:
-                fused = engine.socs_residual_adjoint(m, ks, target, d, g, awork, losses[i:i + 1],
-                                                     plan=plan)
-            if not fused:
-                if img is None:
-                    img = torch.empty_like(theta)
-                engine.socs_image(m, ks, out=img, plan=plan)
-                nat.check(lib.oi_ilt_residual(nat.ptr(img), nat.ptr(target), nat.ptr(d),
-                                              nat.ptr(losses[i:i + 1]), B, npix, nat.ptr(work), st))
-                engine.socs_adjoint(m, d, ks, out=g, plan=plan)
+        if fused:
+            fused = engine.socs_residual_adjoint(m, ks, target, d, g, awork, losses[i:i + 1],
+                                                 plan=plan)
+        if not fused:
+            if img is None:
+                img = torch.empty_like(theta)
+            engine.socs_image(m, ks, out=img, plan=plan)
+            nat.check(lib.oi_ilt_residual(nat.ptr(img), nat.ptr(target), nat.ptr(d),
+                                          nat.ptr(losses[i:i + 1]), B, npix, nat.ptr(work), st))
+            engine.socs_adjoint(m, d, ks, out=g, plan=plan)
         nat.check(lib.oi_ilt_update(nat.ptr(theta), nat.ptr(g), nat.ptr(m), float(lr), B, npix,
                                     nat.ptr(work), nat.ptr(m), st))
     return theta, losses

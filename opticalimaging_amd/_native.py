@@ -102,6 +102,8 @@ SIGNATURES = {
     "oi_ilt_target_band": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
     "oi_socs_band_residual_adjoint": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int, c_void_p,
                                               c_void_p, c_void_p, c_void_p, c_void_p]),
+    "oi_ilt_band_step": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int, c_void_p, c_float,
+                                 c_void_p, c_void_p, c_void_p, c_void_p]),
     "oi_ilt_work_floats": (c_size_t, [c_int]),
     "oi_socs_residual": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int, c_void_p, c_void_p,
                                  c_void_p, c_void_p]),

@@ -34,8 +34,14 @@ int adjoint_band_spectrum(const AdjGeom& g, const float* img, int batch, void* w
 // gI null: the cotangent's band spectrum (the layout adjoint_band_spectrum
 // writes) is already at adjoint_cotangent_spectrum(g, batch, work); not on
 // direct plans, which take the cotangent image itself.
+// chain_theta non-null (needs adjoint_chain_ok): the masks are
+// m = logistic(chain_theta) and grad receives the gradient with respect to
+// theta, grad m (1 - m), with its per-workgroup sums of squares in chain_part
+// ([batch][fp_urows_blocks(ny, nx, batch)]).
+bool adjoint_chain_ok(const AdjGeom& g);
 int launch_adjoint(const AdjGeom& g, const float* gI, const float2* spec, int batch,
                    const float2* kern, int nkern, void* work, float* grad, hipStream_t st,
-                   const float2* bcols = nullptr);
+                   const float2* bcols = nullptr, const float* chain_theta = nullptr,
+                   float* chain_part = nullptr);
 
 }  // namespace oi

@@ -621,10 +621,18 @@ int adjoint_band_spectrum(const AdjGeom& g, const float* img, int batch, void* w
   return launch_spec_cols(Tg, batch, g.ny, g.nx, g.fy, -g.qr, wgr, wgc, out, st);
 }
 
+bool adjoint_chain_ok(const AdjGeom& g) {
+  const int qrm = adj_half(g.k0r, g.wr), qcm = adj_half(g.k0c, g.wc);
+  return fp_image_size_ok(g.ny) && fp_image_size_ok(g.nx) && 2 * qrm + 1 <= g.ny &&
+         2 * qcm + 1 <= g.nx;
+}
+
 int launch_adjoint(const AdjGeom& g, const float* gI, const float2* spec, int batch,
                    const float2* kern, int nkern, void* work, float* grad, hipStream_t st,
-                   const float2* bcols) {
+                   const float2* bcols, const float* chain_theta, float* chain_part) {
   if (!gI && g.direct) return fail(OI_ERR_INVALID, "adjoint: a direct plan needs the cotangent image");
+  if (chain_theta && (!chain_part || !adjoint_chain_ok(g)))
+    return fail(OI_ERR_UNSUPPORTED, "adjoint: the chain-rule form needs the compile-time-size image pass");
   const size_t B = (size_t)batch, K = (size_t)nkern;
   char* w = static_cast<char*>(work);
   const int wgc = 2 * g.qc + 1, wgr = 2 * g.qr + 1;
@@ -712,8 +720,7 @@ int launch_adjoint(const AdjGeom& g, const float* gI, const float2* spec, int ba
     OI_LAUNCH_CHECK("adj_cwin");
   }
   const float scale = 2.0f / ((float)g.lr * (float)g.lc);
-  if (fp_image_size_ok(g.ny) && fp_image_size_ok(g.nx) && 2 * qrm + 1 <= g.ny &&
-      2 * qcm + 1 <= g.nx) {
+  if (adjoint_chain_ok(g)) {
     // register transforms of the image sizes: the kernel sum folded into the
     // band layout of the forward interpolation, then its two kernels
     const size_t nf = B * (qcm + 1) * (2 * qrm + 1);
@@ -732,6 +739,8 @@ int launch_adjoint(const AdjGeom& g, const float* gI, const float2* spec, int ba
     a.Ihat = F;
     a.U = U;
     a.out = grad;
+    a.theta = chain_theta;
+    a.part = chain_part;
     return fp_band_image(a, st);
   }
   const int per = g.wr * g.wc;

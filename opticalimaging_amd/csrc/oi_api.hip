@@ -805,6 +805,45 @@ extern "C" int oi_socs_band_residual_adjoint(oi_plan* p, const float* mask, int 
                         grad_mask, st, bcols);
 }
 
+extern "C" int oi_ilt_band_step(oi_plan* p, float* theta, int batch, const void* kern, int nkern,
+                                const void* tband, float lr, float* loss, void* work,
+                                float* grad_theta, void* stream) {
+  if (!p || !theta || !kern || !tband || !work || !grad_theta)
+    return fail(OI_ERR_INVALID, "ilt_band_step: null argument");
+  if (!band_residual_ok(p) || !adjoint_chain_ok(adj_geom(p)))
+    return fail(OI_ERR_UNSUPPORTED, "ilt_band_step: plan without the band form of the residual");
+  if (batch < 1 || batch > p->g.max_batch) return fail(OI_ERR_INVALID, "batch outside [1, max_batch]");
+  if (nkern < 1 || nkern > p->g.max_kernels)
+    return fail(OI_ERR_INVALID, "ilt_band_step: nkern outside [1, max_kernels]");
+  hipStream_t st = (hipStream_t)stream;
+  const int chan0 = 0;
+  const int slots = p->slots;
+  p->slots = 0;  // stage timing covers the imaging calls only
+  // spectrum of m = logistic(theta), coherent sum up to the image band
+  int rc = spectrum_impl(p, theta, kFieldLogits, 1, &chan0, batch, p->spec, st);
+  if (!rc) rc = coherent_impl(p, p->spec, batch, kern, nkern, nullptr, nullptr, st, true);
+  p->slots = slots;
+  if (rc) return rc;
+  const AdjGeom ag = adj_geom(p);
+  const float2* band = static_cast<const float2*>(tband);
+  const float* tconst = reinterpret_cast<const float*>(static_cast<const char*>(tband) +
+                                                       band_const_offset(p, batch));
+  rc = ilt_band_cotangent(p->Ihat, p->g.qr, p->g.qc, band, tconst,
+                          (float)p->g.ny * (float)p->g.nx, batch,
+                          adjoint_cotangent_spectrum(ag, batch, work), p->part, st);
+  if (rc) return rc;
+  if (loss && (rc = ilt_sum(p->part, batch * ilt_band_blocks(p->g.qr, p->g.qc), loss, st)))
+    return rc;
+  // the loss sums are consumed (same stream): p->part takes the gradient's
+  // norm sums of the adjoint's last pass
+  const float2* bcols = p->g.path == 2 ? p->Bbuf : nullptr;
+  rc = launch_adjoint(ag, nullptr, p->spec, batch, static_cast<const float2*>(kern), nkern, work,
+                      grad_theta, st, bcols, theta, p->part);
+  if (rc) return rc;
+  return ilt_update_theta(theta, grad_theta, lr, batch, (size_t)p->g.ny * p->g.nx, p->part,
+                          fp_urows_blocks(p->g.ny, p->g.nx, batch), st);
+}
+
 extern "C" size_t oi_socs_adjoint_work_bytes(const oi_plan* p, int batch, int nkern) {
   if (!p || batch < 1 || nkern < 1) return 0;
   return adjoint_work_bytes(adj_geom(p), batch, nkern);

@@ -10,6 +10,15 @@ struct Chans {
 };
 
 // Shared launch arguments of the fast path.
+// Internal field kind of the fast path: a real field given as logits, the
+// transformed field being sigmoid(field) (the ILT step's mask parametrisation).
+constexpr int kFieldLogits = 16;
+// sigmoid from the hardware exponential and reciprocal (a few ulp; 0 and 1
+// at the ends), cheap enough to apply to every pixel a pass streams
+__device__ __forceinline__ float logistic(float x) {
+  return __builtin_amdgcn_rcpf(1.f + __expf(-x));
+}
+
 struct FpArgs {
   const void* field;
   int kind, cin, nchan;
@@ -28,6 +37,8 @@ struct FpArgs {
   float2* U;      // [B][qc+1][ny]
   float* out;     // [B][ny][nx]
   const float* target;  // non-null: out = 2 (I - target), part gets the sums of (I - target)^2
+  const float* theta;   // non-null (target null): out = I m (1 - m), m = logistic(theta),
+                        // part gets the sums of out^2
   float* part;          // [B][fp_urows_blocks(ny, nx, B)]
   StageMark mark;
   void* mark_ctx;
@@ -81,6 +92,10 @@ int ilt_band_blocks(int qr, int qc);
 int ilt_band_cotangent(const float2* Ihat, int qr, int qc, const float2* tband,
                        const float* tconst, float n, int batch, float2* specg, float* part,
                        hipStream_t st);
+// theta -= lr gt / rms(gt) per image, the rms from nparts partial sums of
+// gt^2 per image (part [batch][nparts], added in a fixed order)
+int ilt_update_theta(float* theta, const float* gt, float lr, int batch, size_t npix,
+                     const float* part, int nparts, hipStream_t st);
 // tconst[b] = sum_j T[b][j]^2 - n sum_band |tband[b]|^2 (accumulated in
 // float64); scratch: ilt_target_const_scratch(batch) bytes of device memory
 size_t ilt_target_const_scratch(int batch);

@@ -44,7 +44,7 @@ template <class P>
 __global__ __launch_bounds__(kBlock) void k_fp_rows_real(const float* __restrict__ mask, int ny,
                                                          int k0c, int wc,
                                                          const float2* __restrict__ twt,
-                                                         float2* __restrict__ T) {
+                                                         float2* __restrict__ T, int logits) {
   constexpr int N = P::N, TT = P::T, V = P::V, G = groups_per_block<P>();
   extern __shared__ float2 lds[];
   const int g = threadIdx.x / TT, t = threadIdx.x - g * TT;
@@ -58,8 +58,12 @@ __global__ __launch_bounds__(kBlock) void k_fp_rows_real(const float* __restrict
 #pragma unroll
   for (int m = 0; m < V; ++m) {
     const int x = t + TT * m;
-    const float a = (live && r0 < ny) ? __builtin_nontemporal_load(src + (size_t)r0 * N + x) : 0.f;
-    const float b = (live && r1 < ny) ? __builtin_nontemporal_load(src + (size_t)r1 * N + x) : 0.f;
+    float a = (live && r0 < ny) ? __builtin_nontemporal_load(src + (size_t)r0 * N + x) : 0.f;
+    float b = (live && r1 < ny) ? __builtin_nontemporal_load(src + (size_t)r1 * N + x) : 0.f;
+    if (logits) {                    // rows past ny stay zero
+      a = (live && r0 < ny) ? logistic(a) : 0.f;
+      b = (live && r1 < ny) ? logistic(b) : 0.f;
+    }
     v[m] = make_float2(a, b);
   }
   P::template run_live_tab<true>(v, scr, t, twt, true);
@@ -222,7 +226,7 @@ template <class P, int H>
 __global__ __launch_bounds__(kBlock) void k_fp_rows_real_narrow(const float* __restrict__ mask,
                                                                 int ny, int k0c, int wc,
                                                                 const float2* __restrict__ twt,
-                                                                float2* __restrict__ T) {
+                                                                float2* __restrict__ T, int logits) {
   constexpr int N = P::N, TT = P::T, V = P::V, R1 = P::R1, R2 = P::R2, U1 = P::U1;
   constexpr int G = groups_per_block<P>();
   static_assert(P::U2 == 1 && R2 == V && TT == R1 && kBlock % TT == 0 && (TT & (TT - 1)) == 0 &&
@@ -248,8 +252,13 @@ __global__ __launch_bounds__(kBlock) void k_fp_rows_real_narrow(const float* __r
     const int set = blockIdx.x * kRowSets + it;
     if (set >= sets) break;                        // uniform over the block
     float2 v[V];
+    if (logits) {
 #pragma unroll
-    for (int m = 0; m < V; ++m) v[m] = nxt[m];
+      for (int m = 0; m < V; ++m) v[m] = make_float2(logistic(nxt[m].x), logistic(nxt[m].y));
+    } else {
+#pragma unroll
+      for (int m = 0; m < V; ++m) v[m] = nxt[m];
+    }
     if (it + 1 < kRowSets && set + 1 < sets) rows_load_pair<P>(nxt, src, 2 * ((set + 1) * G + g), t);
     float2 pos[H + 1], neg[H];
     narrow_inverse<P, H>(v, scr, t, tw2, pos, neg);   // its first barrier also orders tw2
@@ -815,9 +824,13 @@ __global__ __launch_bounds__(kBlock) void k_fp_ucols(const float2* __restrict__ 
 // Q > 0: the band has qc < Q T, so thread t's inputs x = t + T m can be
 // non-zero only for m < Q or m >= V - Q; the other registers are the constant
 // zero and the first-stage arithmetic on them folds away at compile time.
-// RES: instead of the image I, write 2 (I - target) and the workgroup's sum
-// of (I - target)^2 to part[img * gridDim.x + blockIdx.x] (the residual of a
-// least-squares fit to target, fused into the pass that produces I).
+// MODE 1 (residual): instead of the image I, write 2 (I - target) and the
+// workgroup's sum of (I - target)^2 to part[img * gridDim.x + blockIdx.x]
+// (the residual of a least-squares fit to target, fused into the pass that
+// produces I).  MODE 2 (chain rule): target holds logits theta; write
+// I m (1 - m), m = logistic(theta), and the workgroup's sum of its squares
+// (the gradient with respect to theta of a loss whose gradient with respect
+// to m is I, and its norm).  MODE 0: the image.
 //
 // A block images rsets sets of 2G rows in turn (fp_urows_sets: up to
 // kURowSets while the grid keeps at least kURowMinBlocks workgroups, so
@@ -828,7 +841,7 @@ __global__ __launch_bounds__(kBlock) void k_fp_ucols(const float2* __restrict__ 
 constexpr int kURowSets = 4;
 constexpr int kURowMinBlocks = 512;
 
-template <class P, int Q, bool RES>
+template <class P, int Q, int MODE>
 __global__ __launch_bounds__(kBlock) void k_fp_urows(const float2* __restrict__ U, int ny, int qc,
                                                      const float2* __restrict__ twt,
                                                      float* __restrict__ out,
@@ -906,11 +919,12 @@ __global__ __launch_bounds__(kBlock) void k_fp_urows(const float2* __restrict__ 
       }
     }
     const int ya = set * R + 2 * gg, yb = ya + 1;
-    // residual form: the target rows are loaded before the transform, so their
-    // latency hides behind it (a load issued after the stores would also wait
-    // for them: loads and stores share one in-order counter)
-    float2 tgt[RES ? V : 1];
-    if constexpr (RES) {
+    // residual and chain forms: the target rows are loaded before the
+    // transform, so their latency hides behind it (a load issued after the
+    // stores would also wait for them: loads and stores share one in-order
+    // counter)
+    float2 tgt[MODE ? V : 1];
+    if constexpr (MODE != 0) {
 #pragma unroll
       for (int m = 0; m < V; ++m) {
         const int x = t + TT * m;
@@ -926,7 +940,7 @@ __global__ __launch_bounds__(kBlock) void k_fp_urows(const float2* __restrict__ 
       for (int m = 0; m < V; ++m) {
         const int x = t + TT * m;
         const size_t ia = ((size_t)img * ny + ya) * N + x, ib = ((size_t)img * ny + yb) * N + x;
-        if (RES) {
+        if (MODE == 1) {
           if (ya < ny) {
             const float r = v[m].x - tgt[m].x;
             acc += r * r;
@@ -937,6 +951,24 @@ __global__ __launch_bounds__(kBlock) void k_fp_urows(const float2* __restrict__ 
             acc += r * r;
             __builtin_nontemporal_store(2.f * r, out + ib);
           }
+        } else if (MODE == 2) {
+          // the logits enter the arithmetic only here: otherwise the sigmoid
+          // is scheduled right behind the loads and waits for them before the
+          // transform instead of after it
+          float ta = tgt[m].x, tb = tgt[m].y;
+          asm volatile("" : "+v"(ta), "+v"(tb));
+          if (ya < ny) {
+            const float mv = logistic(ta);
+            const float r = v[m].x * mv * (1.f - mv);
+            acc += r * r;
+            __builtin_nontemporal_store(r, out + ia);
+          }
+          if (yb < ny) {
+            const float mv = logistic(tb);
+            const float r = v[m].y * mv * (1.f - mv);
+            acc += r * r;
+            __builtin_nontemporal_store(r, out + ib);
+          }
         } else {
           if (ya < ny) __builtin_nontemporal_store(v[m].x, out + ia);
           if (yb < ny) __builtin_nontemporal_store(v[m].y, out + ib);
@@ -945,7 +977,7 @@ __global__ __launch_bounds__(kBlock) void k_fp_urows(const float2* __restrict__ 
     }
     __syncthreads();                          // the tile is rewritten by the next set
   }
-  if constexpr (RES) {
+  if constexpr (MODE != 0) {
     __shared__ float red[kBlock / 64];
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
@@ -997,7 +1029,7 @@ static int run_rows(const FpArgs& a, hipStream_t st) {
   // narrow form: H output pairs per thread cover the window [-H T, H T]
   int narrow = 0;
   if constexpr (P::STAGES == 2) {
-    if (a.kind == OI_FIELD_REAL && a.ny % (2 * G) == 0) {
+    if ((a.kind == OI_FIELD_REAL || a.kind == kFieldLogits) && a.ny % (2 * G) == 0) {
       for (int h = 1; h <= 4 && !narrow; ++h)
         if (2 * h < P::V && a.k0c >= -h * P::T && a.k0c + a.wc - 1 <= h * P::T) narrow = h;
     }
@@ -1011,7 +1043,7 @@ static int run_rows(const FpArgs& a, hipStream_t st) {
         int rc = allow(kernel, lds_n);
         if (rc) return rc;
         hipLaunchKernelGGL(kernel, grid, dim3(kBlock), lds_n, st, static_cast<const float*>(a.field),
-                           a.ny, a.k0c, a.wc, a.tw_x, a.T);
+                           a.ny, a.k0c, a.wc, a.tw_x, a.T, (int)(a.kind == kFieldLogits));
         return OI_OK;
       };
       int rc = narrow == 1   ? go(k_fp_rows_real_narrow<P, 1>)
@@ -1020,12 +1052,13 @@ static int run_rows(const FpArgs& a, hipStream_t st) {
                              : go(k_fp_rows_real_narrow<P, 4>);
       if (rc) return rc;
     }
-  } else if (a.kind == OI_FIELD_REAL) {
+  } else if (a.kind == OI_FIELD_REAL || a.kind == kFieldLogits) {
     int rc = allow(k_fp_rows_real<P>, lds);
     if (rc) return rc;
     dim3 grid((a.ny + 2 * G - 1) / (2 * G), a.batch);
     hipLaunchKernelGGL(k_fp_rows_real<P>, grid, dim3(kBlock), lds, st,
-                       static_cast<const float*>(a.field), a.ny, a.k0c, a.wc, a.tw_x, a.T);
+                       static_cast<const float*>(a.field), a.ny, a.k0c, a.wc, a.tw_x, a.T,
+                       (int)(a.kind == kFieldLogits));
   } else {
     int rc = allow(k_fp_rows_c64<P>, lds);
     if (rc) return rc;
@@ -1089,26 +1122,32 @@ static int run_urows(const FpArgs& a, hipStream_t st) {
   const size_t lds = rows_lds<P>() + (size_t)(a.qc + 1) * (2 * G + 1) * sizeof(float2);
   const int rsets = fp_urows_sets(a.ny, P::N, a.batch);
   dim3 grid(fp_urows_blocks(a.ny, P::N, a.batch), a.batch);
-  auto go = [&](auto kernel) -> int {
+  auto go = [&](auto kernel, const float* side) -> int {
     int rc = allow(kernel, lds);
     if (rc) return rc;
-    hipLaunchKernelGGL(kernel, grid, dim3(kBlock), lds, st, a.U, a.ny, a.qc, a.tw_x, a.out,
-                       a.target, a.part, rsets);
+    hipLaunchKernelGGL(kernel, grid, dim3(kBlock), lds, st, a.U, a.ny, a.qc, a.tw_x, a.out, side,
+                       a.part, rsets);
     OI_LAUNCH_CHECK("fp_urows");
     return OI_OK;
   };
   // registers that can hold band values: Q = ceil((qc + 1) / T) at each end
   const int q = (a.qc + 1 + P::T - 1) / P::T;
   if (a.target) {
-    if (q <= 2 && 2 * 2 <= P::V) return go(k_fp_urows<P, 2, true>);
-    if (q <= 3 && 2 * 3 <= P::V) return go(k_fp_urows<P, 3, true>);
-    if (q <= 4 && 2 * 4 < P::V) return go(k_fp_urows<P, 4, true>);
-    return go(k_fp_urows<P, 0, true>);
+    if (q <= 2 && 2 * 2 <= P::V) return go(k_fp_urows<P, 2, 1>, a.target);
+    if (q <= 3 && 2 * 3 <= P::V) return go(k_fp_urows<P, 3, 1>, a.target);
+    if (q <= 4 && 2 * 4 < P::V) return go(k_fp_urows<P, 4, 1>, a.target);
+    return go(k_fp_urows<P, 0, 1>, a.target);
   }
-  if (q <= 2 && 2 * 2 <= P::V) return go(k_fp_urows<P, 2, false>);
-  if (q <= 3 && 2 * 3 <= P::V) return go(k_fp_urows<P, 3, false>);
-  if (q <= 4 && 2 * 4 < P::V) return go(k_fp_urows<P, 4, false>);
-  return go(k_fp_urows<P, 0, false>);
+  if (a.theta) {
+    if (q <= 2 && 2 * 2 <= P::V) return go(k_fp_urows<P, 2, 2>, a.theta);
+    if (q <= 3 && 2 * 3 <= P::V) return go(k_fp_urows<P, 3, 2>, a.theta);
+    if (q <= 4 && 2 * 4 < P::V) return go(k_fp_urows<P, 4, 2>, a.theta);
+    return go(k_fp_urows<P, 0, 2>, a.theta);
+  }
+  if (q <= 2 && 2 * 2 <= P::V) return go(k_fp_urows<P, 2, 0>, nullptr);
+  if (q <= 3 && 2 * 3 <= P::V) return go(k_fp_urows<P, 3, 0>, nullptr);
+  if (q <= 4 && 2 * 4 < P::V) return go(k_fp_urows<P, 4, 0>, nullptr);
+  return go(k_fp_urows<P, 0, 0>, nullptr);
 }
 
 // The one-pass kernel serves coarse sizes with a block size in the table when

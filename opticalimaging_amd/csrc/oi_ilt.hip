@@ -115,7 +115,8 @@ __global__ __launch_bounds__(kIltThreads) void k_ilt_gnorm(const float* __restri
   if (threadIdx.x == 0) part[(size_t)blockIdx.y * kIltBlocks + blockIdx.x] = s;
 }
 
-__device__ __forceinline__ float sigmoid(float x) { return 1.f / (1.f + expf(-x)); }
+// the sigmoid the spectrum and gradient passes apply to logits (logistic)
+__device__ __forceinline__ float sigmoid(float x) { return logistic(x); }
 
 // theta -= (lr / rms) dL/dm m (1 - m), rms = sqrt(sum / npix) + 1e-30 of the
 // image's partial sums (each workgroup adds them in the same order);
@@ -160,6 +161,39 @@ __global__ __launch_bounds__(kIltThreads) void k_ilt_update(float* theta, const 
       theta[base + i] = t;
       if (m_next) m_next[base + i] = sigmoid(t);
     }
+  }
+}
+
+// theta -= (lr / rms) gt per image, rms = sqrt(sum / npix) + 1e-30 of the
+// image's nparts partial sums of gt^2 (each workgroup adds them in the same
+// order); gt is already the gradient with respect to theta.
+template <bool VEC>
+__global__ __launch_bounds__(kIltThreads) void k_ilt_update_theta(float* theta,
+                                                                 const float* __restrict__ gt,
+                                                                 float lr, size_t npix,
+                                                                 const float* __restrict__ part,
+                                                                 int nparts) {
+  __shared__ float coef;
+  if (threadIdx.x < 64) {
+    double s = 0.0;                             // lane l adds l, l + 64, ... in order
+    for (int i = threadIdx.x; i < nparts; i += 64) s += part[(size_t)blockIdx.y * nparts + i];
+    s = wave_sum(s);
+    const double rms = sqrt(s) / sqrt((double)npix) + 1e-30;
+    if (threadIdx.x == 0) coef = (float)(lr / rms);
+  }
+  __syncthreads();
+  const float c = coef;
+  const size_t base = (size_t)blockIdx.y * npix;
+  const size_t stride = (size_t)kIltBlocks * kIltThreads;
+  if (VEC) {
+    const size_t n4 = npix / 4;
+    f4v* t4 = reinterpret_cast<f4v*>(theta + base);
+    const f4v* g4 = reinterpret_cast<const f4v*>(gt + base);
+    for (size_t i = blockIdx.x * (size_t)kIltThreads + threadIdx.x; i < n4; i += stride)
+      t4[i] = t4[i] - c * g4[i];
+  } else {
+    for (size_t i = blockIdx.x * (size_t)kIltThreads + threadIdx.x; i < npix; i += stride)
+      theta[base + i] -= c * gt[base + i];
   }
 }
 
@@ -305,6 +339,19 @@ int ilt_band_cotangent(const float2* Ihat, int qr, int qc, const float2* tband,
   hipLaunchKernelGGL(k_ilt_band_cotangent, grid, dim3(kIltThreads), 0, st, Ihat, qr, qc, tband,
                      tconst, n, specg, part);
   OI_LAUNCH_CHECK("ilt_band_cotangent");
+  return OI_OK;
+}
+
+int ilt_update_theta(float* theta, const float* gt, float lr, int batch, size_t npix,
+                     const float* part, int nparts, hipStream_t st) {
+  const dim3 grid(kIltBlocks, batch);
+  if (npix % 4 == 0 && aligned16(theta) && aligned16(gt))
+    hipLaunchKernelGGL(k_ilt_update_theta<true>, grid, dim3(kIltThreads), 0, st, theta, gt, lr, npix,
+                       part, nparts);
+  else
+    hipLaunchKernelGGL(k_ilt_update_theta<false>, grid, dim3(kIltThreads), 0, st, theta, gt, lr,
+                       npix, part, nparts);
+  OI_LAUNCH_CHECK("ilt_update_theta");
   return OI_OK;
 }
 

@@ -624,6 +624,46 @@ def socs_band_residual_adjoint(masks: torch.Tensor, ks: KernelSet, tband: torch.
                                                 nat.ptr(grad), nat.stream()))
 
 
+def ilt_band_step(theta: torch.Tensor, ks: KernelSet, tband: torch.Tensor, lr: float,
+                  grad: torch.Tensor, work: torch.Tensor, loss: torch.Tensor | None = None,
+                  plan: Plan | None = None) -> None:
+    """One normalised gradient step on the logits theta [B, ny, nx] (updated
+    in place) against the targets `tband` was made from (oi_ilt_band_step):
+    loss[0] = sum (I(sigmoid(theta)) - T)^2 before the step, grad = the
+    gradient with respect to theta, theta -= lr grad / rms(grad) per image."""
+    if not isinstance(theta, torch.Tensor) or theta.dim() != 3:
+        raise ValueError("theta must be a (B, ny, nx) device tensor")
+    B = int(theta.shape[0])
+    shape = (B, ks.ny, ks.nx)
+    for name, t in (("theta", theta), ("grad", grad)):
+        _device_tensor(name, t, torch.float32, shape)
+    if loss is not None and not (loss.is_cuda and loss.dtype == torch.float32 and loss.numel() >= 1
+                                 and loss.is_contiguous()):
+        raise ValueError("loss must be a contiguous float32 device tensor with one element")
+    if B == 0:
+        if loss is not None:
+            loss.zero_()
+        return
+    ks = _real_set(ks, True)
+    if plan is None:
+        plan = plan_for(ks, 1, B)
+    _check_plan(plan, ks, 1, B)
+    lib = nat.load()
+    nb = int(lib.oi_ilt_target_band_bytes(plan.handle, B))
+    if nb == 0:
+        raise ValueError("the plan has no band form of the residual")
+    if not (isinstance(tband, torch.Tensor) and tband.is_cuda and tband.dtype == torch.uint8
+            and tband.is_contiguous() and tband.numel() >= nb):
+        raise ValueError(f"tband must be an ilt_target_band tensor of {nb} bytes")
+    need = int(lib.oi_socs_adjoint_work_bytes(plan.handle, B, ks.K))
+    if not (isinstance(work, torch.Tensor) and work.is_cuda and work.dtype == torch.uint8
+            and work.is_contiguous() and work.numel() >= need):
+        raise ValueError(f"work must be a contiguous uint8 device tensor of at least {need} bytes")
+    nat.check(lib.oi_ilt_band_step(plan.handle, nat.ptr(theta), B, nat.ptr(ks.win), ks.K,
+                                   nat.ptr(tband), float(lr), nat.ptr(loss), nat.ptr(work),
+                                   nat.ptr(grad), nat.stream()))
+
+
 def adjoint_work(ks: KernelSet, batch: int, plan: Plan | None = None) -> torch.Tensor:
     """Device scratch for socs_adjoint / socs_residual_adjoint on `plan`."""
     ks = _real_set(ks, True)

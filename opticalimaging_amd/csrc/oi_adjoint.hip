@@ -307,12 +307,18 @@ __global__ __launch_bounds__(kRegBlock) void k_adj_cwin_reg(const float2* __rest
                                                             const float2* __restrict__ kern, int nk,
                                                             int k0r, int wr, int wc,
                                                             const float2* __restrict__ twt,
-                                                            float2* __restrict__ Ak) {
+                                                            float2* __restrict__ Ak, int batch) {
   constexpr int L = P::N, T = P::T, V = P::V, G = kRegBlock / T;
   extern __shared__ float2 lds[];
+  // items (column block, kernel, image), column blocks fastest, in XCD order:
+  // a row of Pc is split over the column blocks of one (image, kernel), and
+  // the lines at their edges are read by two of them
+  const int nxb = (wc + G - 1) / G;
+  int w;
+  if (!xcd_item(blockIdx.x, nxb * nk * batch, w)) return;   // uniform over the block
+  const int xb = w % nxb, k = (w / nxb) % nk, b = w / (nxb * nk);
   const int g = threadIdx.x / T, t = threadIdx.x - g * T;
-  const int k = blockIdx.y, b = blockIdx.z;
-  const int kc = blockIdx.x * G + g;
+  const int kc = xb * G + g;
   const bool on = g < G && kc < wc;
   const float2* pb = Pc + ((size_t)b * nk + k) * L * wc + (on ? kc : 0);
   float2 v[V];
@@ -426,9 +432,10 @@ static int adj_cwin_reg(int L, dim3 grid_y_z, const float2* Pc, const float2* ke
     constexpr int G = kRegBlock / P::T;                                                        \
     int rc = allow_lds_adj(k_adj_cwin_reg<P>, adj_reg_lds<P>());                               \
     if (rc) return rc;                                                                         \
-    dim3 grid((wc + G - 1) / G, grid_y_z.y, grid_y_z.z);                                       \
-    hipLaunchKernelGGL(k_adj_cwin_reg<P>, grid, dim3(kRegBlock), adj_reg_lds<P>(), st, Pc,     \
-                       kern, nk, k0r, wr, wc, twt, Ak);                                        \
+    const int items = ((wc + G - 1) / G) * (int)grid_y_z.y * (int)grid_y_z.z;                  \
+    hipLaunchKernelGGL(k_adj_cwin_reg<P>, dim3(xcd_grid(items)), dim3(kRegBlock),              \
+                       adj_reg_lds<P>(), st, Pc, kern, nk, k0r, wr, wc, twt, Ak,               \
+                       (int)grid_y_z.z);                                                       \
     OI_LAUNCH_CHECK("adj_cwin_reg");                                                           \
     return OI_OK;                                                                              \
   }

@@ -33,6 +33,24 @@ __host__ __device__ __forceinline__ int pmod(int a, int m) {
   return r < 0 ? r + m : r;
 }
 
+// ------------------------------------------------------------ XCD order --
+// Workgroups of a 1-D grid are dispatched round-robin over the 8 XCDs
+// (workgroup p on XCD p mod 8), each with its own L2.  xcd_item gives
+// workgroup p the work item w such that every XCD takes one contiguous run of
+// the items, in order: neighbouring items, whose edges share cache lines, run
+// on one L2 at about the same time instead of each fetching those lines from
+// HBM.  The grid holds xcd_grid(total) workgroups; xcd_item is false for the
+// few past the last item.
+constexpr int kXcdCount = 8;
+__host__ __device__ __forceinline__ int xcd_grid(int total) {
+  return kXcdCount * ((total + kXcdCount - 1) / kXcdCount);
+}
+__device__ __forceinline__ bool xcd_item(int p, int total, int& w) {
+  const int per = (total + kXcdCount - 1) / kXcdCount;
+  w = (p % kXcdCount) * per + p / kXcdCount;
+  return w < total;
+}
+
 // ------------------------------------------------------------ host errors --
 void set_error(int code, const std::string& msg);
 int fail(int code, const std::string& msg);

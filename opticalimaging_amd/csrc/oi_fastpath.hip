@@ -683,16 +683,24 @@ template <class PL>
 __global__ __launch_bounds__(kBlock) void k_fp_brows(const float2* __restrict__ B, int nkc, int k0c,
                                                      int wc, int qc, int generic_layout,
                                                      const float2* __restrict__ twt,
-                                                     float2* __restrict__ R) {
+                                                     float2* __restrict__ R, int batch,
+                                                     int groups) {
   constexpr int L = PL::N, TL = PL::T, VL = PL::V, G = groups_per_block<PL>();
   constexpr int TS = G + 1;                // tile row stride (bank spread)
   static_assert(VL <= 32, "row mask holds VL bits");
   extern __shared__ float2 lds[];
+  // items (row group, image, kernel group), row groups fastest, in XCD order:
+  // a column of B is split over the row groups of one (image, kernel group),
+  // and where 8 G bytes do not divide the line, the lines at the edges are
+  // read by two of them
+  constexpr int NXR = (L + G - 1) / G;
+  int w;
+  if (!xcd_item(blockIdx.x, NXR * batch * groups, w)) return;   // uniform over the block
+  const int xr = w % NXR, img = (w / NXR) % batch, grp = w / (NXR * batch);
   float2* tile = lds;                      // [wc][TS]
   float2* scr = lds;                       // transform scratch, over the tile once it is read
   const int g = threadIdx.x / TL, t = threadIdx.x - g * TL;
-  const int img = blockIdx.y, grp = blockIdx.z, groups = gridDim.z;
-  const int r0 = blockIdx.x * G;
+  const int r0 = xr * G;
   const int row = r0 + g;
   const bool live = g < G && row < L;
   unsigned rmask = 0;
@@ -1225,9 +1233,10 @@ static int run_two_pass(const FpArgs& a, const FpCoh& c, float2* B, int generic_
   rc = allow(k_fp_brows<PL>, lds2);
   if (rc) return rc;
   if (generic_layout && c.groups != 1) return fail(OI_ERR_INVALID, "two-pass path: one group in the generic layout");
-  dim3 g2((PL::N + G - 1) / G, a.batch, c.groups);
-  hipLaunchKernelGGL(k_fp_brows<PL>, g2, dim3(kBlock), lds2, st, B, c.nkern * c.nchan, a.k0c, a.wc,
-                     a.qc, generic_layout, a.tw_l, c.R);
+  const int items = ((PL::N + G - 1) / G) * a.batch * c.groups;
+  hipLaunchKernelGGL(k_fp_brows<PL>, dim3(xcd_grid(items)), dim3(kBlock), lds2, st, B,
+                     c.nkern * c.nchan, a.k0c, a.wc, a.qc, generic_layout, a.tw_l, c.R, a.batch,
+                     c.groups);
   OI_LAUNCH_CHECK("fp_brows");
   return OI_OK;
 }

@@ -20,7 +20,7 @@ import numpy as np
 import torch
 
 from . import _native as nat
-from . import engine
+from . import engine, fingerprint
 
 __all__ = ["optics_fingerprint", "save_kernel_set", "load_kernel_set", "cached_kernel_set"]
 
@@ -30,44 +30,11 @@ _DERIVED = {"source_data"}
 
 
 def _canon(v, depth=0):
-    """A JSON-able form of a parameter value; arrays enter by shape, dtype
-    and a digest of their bytes."""
-    if depth > 4:
-        raise ValueError("parameter objects nest too deeply to fingerprint")
-    if v is None or isinstance(v, (bool, int, str)):
-        return v
-    if isinstance(v, float):
-        return repr(v)
-    if isinstance(v, complex):
-        return [repr(v.real), repr(v.imag)]
-    if isinstance(v, torch.Tensor):
-        v = v.detach().cpu().contiguous().numpy()
-    if isinstance(v, np.ndarray):
-        a = np.ascontiguousarray(v)
-        return {"shape": list(a.shape), "dtype": str(a.dtype),
-                "sha256": hashlib.sha256(a.tobytes()).hexdigest()}
-    if isinstance(v, np.generic):
-        return _canon(v.item(), depth)
-    if isinstance(v, (list, tuple)):
-        return [_canon(x, depth + 1) for x in v]
-    if isinstance(v, dict):
-        return {str(k): _canon(x, depth + 1) for k, x in sorted(v.items())}
-    return _params(v, depth + 1)
+    return fingerprint.canon(v, depth)
 
 
 def _params(obj, depth=0):
-    """Public, non-callable attributes of a parameter object (instance and
-    class attributes, as the reference's Numerics keeps its settings on the
-    class)."""
-    out = {"__class__": type(obj).__name__}
-    for k in sorted(dir(obj)):
-        if k.startswith("_") or k in _DERIVED:
-            continue
-        v = getattr(obj, k)
-        if callable(v):
-            continue
-        out[k] = _canon(v, depth)
-    return out
+    return fingerprint.params(obj, depth, exclude=_DERIVED)
 
 
 def optics_fingerprint(source, projector, numerics, xypitch, focus, Nfg) -> str:

@@ -374,10 +374,15 @@ static FpArgs fast_args(oi_plan* p, int batch, hipStream_t st) {
 static int spectrum_impl(oi_plan* p, const void* field, int kind, int cin, const int* chans,
                          int batch, void* spec, hipStream_t st);
 
-// One pass over the field; each wave ORs its lanes' bits together and one
-// lane per wave updates the mask.
-__global__ void k_field_flags(const void* __restrict__ field, int cplx, long long npix, int cin,
-                              unsigned int* __restrict__ mask) {
+// One pass over the field; each workgroup ORs its lanes' bits together and
+// one lane per workgroup updates the mask (a few hundred atomics on one word
+// instead of one per wave).
+constexpr int kFlagBlocks = 256;
+
+__global__ __launch_bounds__(256) void k_field_flags(const void* __restrict__ field, int cplx,
+                                                     long long npix, int cin,
+                                                     unsigned int* __restrict__ mask) {
+  __shared__ unsigned int red[4];
   unsigned int bits = 0;
   const long long stride = (long long)gridDim.x * blockDim.x;
   for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < npix; p += stride) {
@@ -393,7 +398,12 @@ __global__ void k_field_flags(const void* __restrict__ field, int cplx, long lon
   }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) bits |= __shfl_xor(bits, o);
-  if ((threadIdx.x & 63) == 0 && bits) atomicOr(mask, bits);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = bits;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned int all = red[0] | red[1] | red[2] | red[3];
+    if (all) atomicOr(mask, all);
+  }
 }
 
 extern "C" int oi_field_flags(const void* field, int kind, long long npix, int cin, unsigned int* mask,
@@ -404,7 +414,7 @@ extern "C" int oi_field_flags(const void* field, int kind, long long npix, int c
     return fail(OI_ERR_INVALID, "field_flags: unknown field kind");
   if (npix == 0) return OI_OK;
   long long blocks = (npix + 255) / 256;
-  if (blocks > 2048) blocks = 2048;
+  if (blocks > kFlagBlocks) blocks = kFlagBlocks;
   hipLaunchKernelGGL(k_field_flags, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, field,
                      kind == OI_FIELD_C64_CHLAST ? 1 : 0, npix, cin, mask);
   OI_LAUNCH_CHECK("field_flags");

@@ -795,6 +795,68 @@ __global__ __launch_bounds__(kBandBlock) void k_fp_band(const float2* __restrict
   }
 }
 
+// The same band for many kernel groups (small batches, whose coherent sum
+// splits the kernels over up to 64 groups): one order q per workgroup of
+// kBandWide threads.  The group sum is spread over the threads (S = kBandWide /
+// L slices of the groups, each slice summed in order, then the slices in
+// order), so it no longer runs as `groups` dependent loads per thread of a
+// handful of workgroups.
+constexpr int kBandWide = 256;
+constexpr int kBandWideMinGroups = 9;   // fewer groups: k_fp_band
+
+template <class PL>
+__global__ __launch_bounds__(kBandWide) void k_fp_band_wide(const float2* __restrict__ R,
+                                                            int groups, int qc, int qrm,
+                                                            float scale,
+                                                            const float2* __restrict__ twt,
+                                                            float2* __restrict__ Ihat) {
+  constexpr int L = PL::N, TL = PL::T, VL = PL::V;
+  constexpr int S = kBandWide / L > 0 ? kBandWide / L : 1;   // group slices
+  static_assert(TL <= kBandWide, "one transform group");
+  extern __shared__ float2 lds[];
+  float2* part = lds;                    // [S][L]
+  float2* sum = lds + S * L;             // [L]
+  float2* scr = sum + L;                 // the transform's exchange
+  const int img = blockIdx.y, q = blockIdx.x;
+  const int nq = qc + 1;
+  const float2* src = R + ((size_t)img * groups * nq + q) * L;
+  for (int i = threadIdx.x; i < S * L; i += kBandWide) {
+    const int gs = i / L, t = i - gs * L;
+    float2 a = make_float2(0.f, 0.f);
+    for (int k = gs; k < groups; k += S) {
+      const float2 x = src[(size_t)k * nq * L + t];
+      a.x += x.x;
+      a.y += x.y;
+    }
+    part[i] = a;
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < L; t += kBandWide) {
+    float2 a = part[t];
+    for (int gs = 1; gs < S; ++gs) {
+      a.x += part[gs * L + t].x;
+      a.y += part[gs * L + t].y;
+    }
+    sum[t] = a;
+  }
+  __syncthreads();
+  const int t = threadIdx.x;
+  const bool live = t < TL;
+  float2 v[VL];
+#pragma unroll
+  for (int m = 0; m < VL; ++m) v[m] = live ? sum[t + TL * m] : make_float2(0.f, 0.f);
+  PL::template run_live_tab<true>(v, scr, live ? t : 0, twt, live);
+  if (!live) return;
+  const int nb = 2 * qrm + 1;
+  float2* dst = Ihat + ((size_t)img * nq + q) * nb;
+#pragma unroll
+  for (int m = 0; m < VL; ++m) {
+    const int x = t + TL * m;
+    const int qr = x <= qrm ? x : x - L;
+    if (qr >= -qrm && qr <= qrm) dst[qr + qrm] = make_float2(v[m].x * scale, v[m].y * scale);
+  }
+}
+
 // U[img][q][y] = sum_qr Ihat[img][q][qr] e^{-2 pi i qr y / ny}
 // Q > 0: qrm < Q T, so only registers m < Q and m >= V - Q can hold band
 // values (as in k_fp_urows).
@@ -1195,6 +1257,17 @@ static int run_coherent(const FpArgs& a, const FpCoh& c, hipStream_t st) {
 
 template <class PL>
 static int run_band(const FpArgs& a, int groups, hipStream_t st) {
+  if (groups >= kBandWideMinGroups) {
+    constexpr int L = PL::N, S = kBandWide / L > 0 ? kBandWide / L : 1;
+    const size_t lds = ((size_t)S * L + L + PL::LDS) * sizeof(float2);
+    int rc = allow(k_fp_band_wide<PL>, lds);
+    if (rc) return rc;
+    const float scale = 1.0f / ((float)PL::N * (float)PL::N);
+    hipLaunchKernelGGL(k_fp_band_wide<PL>, dim3(a.qc + 1, a.batch), dim3(kBandWide), lds, st, a.R,
+                       groups, a.qc, a.qr, scale, a.tw_l, a.Ihat);
+    OI_LAUNCH_CHECK("fp_band_wide");
+    return OI_OK;
+  }
   constexpr int G = kBandBlock / PL::T;
   const size_t lds = (size_t)((kBandBlock + PL::T - 1) / PL::T) * PL::LDS * sizeof(float2);
   int rc = allow(k_fp_band<PL>, lds);

@@ -2,14 +2,17 @@
 on the device, the oi_optics record, shifted-pupil kernels and grid sizes."""
 from __future__ import annotations
 
+import collections
 import ctypes
 import math
+import threading
 from dataclasses import dataclass
 
 import numpy as np
 import torch
 
 from . import _native as nat
+from . import fingerprint
 
 
 def imaging_mode(numerics) -> bool:
@@ -58,10 +61,46 @@ class SourceSet:
         return self.extent
 
 
+# Source sets by the source's parameters (fingerprint.digest) and device:
+# Calc_SourceSimple and the upload run once per distinct source, and a hit
+# restores the state the call leaves on the source.  The cached device
+# tensors are shared: callers read them and never modify them in place.
+_SOURCE_CACHE = collections.OrderedDict()
+_SOURCE_CACHE_SIZE = 8
+_SOURCE_LOCK = threading.Lock()
+
+
 def source_set(source) -> SourceSet:
     """Calc_SourceSimple (Source.py:47-54) plus polarisation, moved to the
-    device.  Leaves ``source.source_data`` populated as the reference does."""
+    device.  Leaves ``source.source_data`` populated as the reference does.
+    Repeated calls with unchanged source parameters reuse the device set."""
     dev = nat.device()
+    key = (fingerprint.digest(source, exclude=("source_data",)), dev.index)
+    with _SOURCE_LOCK:
+        hit = _SOURCE_CACHE.get(key)
+        if hit is not None:
+            _SOURCE_CACHE.move_to_end(key)
+    if hit is not None:
+        holder, X, Y, V, sset = hit
+        # the state Calc_SourceSimple leaves: source.source_data is the
+        # SourceData record (class attributes, Source.py:293-295) holding the
+        # lit points
+        holder.X, holder.Y, holder.Value = X.clone(), Y.clone(), V.clone()
+        source.source_data = holder
+        return sset
+    sset, sd = _source_set_uncached(source, dev)
+    # cache only the reference's record form: the SourceData class itself,
+    # which every Source shares, so a hit hands no object of another source
+    if isinstance(sd, type):
+        entry = (sd, sd.X.clone(), sd.Y.clone(), sd.Value.clone(), sset)
+        with _SOURCE_LOCK:
+            _SOURCE_CACHE[key] = entry
+            while len(_SOURCE_CACHE) > _SOURCE_CACHE_SIZE:
+                _SOURCE_CACHE.popitem(last=False)
+    return sset
+
+
+def _source_set_uncached(source, dev):
     # host-side selection: the drop-in leaves source.source_data on the host
     # as the reference does (Source.Calc_SourceSimple(device=...) gives the
     # same lit set on the device for callers that keep it there)
@@ -80,8 +119,9 @@ def source_set(source) -> SourceSet:
         PX = torch.as_tensor(PX).to(torch.complex128)
         PY = torch.as_tensor(PY).to(torch.complex128)
     extent = (float(X.min()), float(X.max()), float(Y.min()), float(Y.max()))
-    return SourceSet(X.to(dev), Y.to(dev), V.to(dev), PX.to(dev).contiguous(), PY.to(dev).contiguous(),
-                     extent, float(V.sum()))
+    sset = SourceSet(X.to(dev), Y.to(dev), V.to(dev), PX.to(dev).contiguous(),
+                     PY.to(dev).contiguous(), extent, float(V.sum()))
+    return sset, source.source_data
 
 
 def optics_record(source, projector, focus, pitch_r, pitch_c, vector) -> nat.Optics:

@@ -369,7 +369,8 @@ int oi_socs_band_residual_adjoint(oi_plan* plan, const float* mask, int batch, c
 /* oi_ilt_band_step: one whole normalised gradient step on the logits,
  *   m = sigmoid(theta) [batch][ny][nx] (device, updated in place): the loss
  *   (device float, may be null) and dL/dm as oi_socs_band_residual_adjoint,
- *   grad_theta = dL/dm m (1 - m) (device [batch][ny][nx], written), then per
+ *   grad_theta = dL/dm m (1 - m) (device [batch][ny][nx], written; not
+ *   theta itself), then per
  *   image theta -= lr grad_theta / (sqrt(sum grad_theta^2 / npix) + 1e-30),
  *   as oi_ilt_update.  m is never stored: the spectrum pass and the
  *   gradient's last pass take the sigmoid of theta as they load it, and the

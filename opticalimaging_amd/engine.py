@@ -637,6 +637,8 @@ def ilt_band_step(theta: torch.Tensor, ks: KernelSet, tband: torch.Tensor, lr: f
     shape = (B, ks.ny, ks.nx)
     for name, t in (("theta", theta), ("grad", grad)):
         _device_tensor(name, t, torch.float32, shape)
+    if B > 0 and theta.data_ptr() == grad.data_ptr():
+        raise ValueError("grad must not be theta itself")
     if loss is not None and not (loss.is_cuda and loss.dtype == torch.float32 and loss.numel() >= 1
                                  and loss.is_contiguous()):
         raise ValueError("loss must be a contiguous float32 device tensor with one element")

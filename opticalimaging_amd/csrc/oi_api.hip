@@ -820,6 +820,7 @@ extern "C" int oi_ilt_band_step(oi_plan* p, float* theta, int batch, const void*
                                 float* grad_theta, void* stream) {
   if (!p || !theta || !kern || !tband || !work || !grad_theta)
     return fail(OI_ERR_INVALID, "ilt_band_step: null argument");
+  if (grad_theta == theta) return fail(OI_ERR_INVALID, "ilt_band_step: grad_theta must not be theta");
   if (!band_residual_ok(p) || !adjoint_chain_ok(adj_geom(p)))
     return fail(OI_ERR_UNSUPPORTED, "ilt_band_step: plan without the band form of the residual");
   if (batch < 1 || batch > p->g.max_batch) return fail(OI_ERR_INVALID, "batch outside [1, max_batch]");

@@ -273,8 +273,11 @@ __device__ __forceinline__ void tcc_panel_store(unsigned short* lds,
 // the mirrored tile one row of the conjugate transpose, instead of four
 // 128-byte pieces of four rows.  It shares the LDS (dynamic, TccTile::kLds
 // bytes) of the operand panels.
+// Four waves per SIMD: the register budget (128) that lets four workgroups
+// share a CU (their 33 KB of LDS each fit four times); 0.346 ms at M = 10201
+// against 0.358-0.370 ms at the 132 registers the compiler takes unbounded.
 template <int TB>
-__global__ __launch_bounds__(256) void k_tcc_bf16(const unsigned short* __restrict__ planes, int M,
+__global__ __launch_bounds__(256, 4) void k_tcc_bf16(const unsigned short* __restrict__ planes, int M,
                                                   int Kp, int nt, double alpha,
                                                   float2* __restrict__ tcc) {
   using TT = TccTile<TB>;

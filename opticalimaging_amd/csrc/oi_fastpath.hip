@@ -679,8 +679,10 @@ __global__ __launch_bounds__(kBlock) void k_fp_bcols(const float2* __restrict__ 
 // Blocks (row group, img, grp) sum the (kernel, channel) pairs kc = grp,
 // grp + groups, ... into their own partial R[img][grp] (groups = 1 for the
 // generic layout).
+// Four waves per SIMD (at most 128 registers): at coarse 256 four workgroups
+// then share a CU, as their 34 KB of LDS allow (ILT step 0.355 -> 0.348 ms).
 template <class PL>
-__global__ __launch_bounds__(kBlock) void k_fp_brows(const float2* __restrict__ B, int nkc, int k0c,
+__global__ __launch_bounds__(kBlock, 4) void k_fp_brows(const float2* __restrict__ B, int nkc, int k0c,
                                                      int wc, int qc, int generic_layout,
                                                      const float2* __restrict__ twt,
                                                      float2* __restrict__ R, int batch,

@@ -325,10 +325,16 @@ def test_band_residual_needs_the_band_form(gpu):
         engine.ilt_band_step(T, ks, torch.zeros(16, dtype=torch.uint8, device="cuda"), 0.5,
                              torch.empty_like(T), work, plan=plan)
     # the library itself refuses before launching anything
+    G = torch.empty_like(T)
+    rc = nat.load().oi_ilt_band_step(plan.handle, nat.ptr(T), 2, nat.ptr(ks.win), ks.K,
+                                     nat.ptr(work), 0.5, None, nat.ptr(work), nat.ptr(G),
+                                     nat.stream())
+    assert rc == nat.OI_ERR_UNSUPPORTED
+    # and a gradient buffer that is theta itself is refused as invalid
     rc = nat.load().oi_ilt_band_step(plan.handle, nat.ptr(T), 2, nat.ptr(ks.win), ks.K,
                                      nat.ptr(work), 0.5, None, nat.ptr(work), nat.ptr(T),
                                      nat.stream())
-    assert rc == nat.OI_ERR_UNSUPPORTED
+    assert rc == nat.OI_ERR_INVALID
 
 
 def test_socs_residual_reports_generic_plans(gpu):

@@ -781,22 +781,19 @@ extern "C" int oi_ilt_target_band(oi_plan* p, const float* target, int batch, vo
                           tconst, st);
 }
 
-extern "C" int oi_socs_band_residual_adjoint(oi_plan* p, const float* mask, int batch,
-                                             const void* kern, int nkern, const void* tband,
-                                             float* loss, void* work, float* grad_mask,
-                                             void* stream) {
-  if (!p || !mask || !kern || !tband || !work || !grad_mask)
-    return fail(OI_ERR_INVALID, "socs_band_residual_adjoint: null argument");
-  if (!band_residual_ok(p))
-    return fail(OI_ERR_UNSUPPORTED, "socs_band_residual_adjoint: plan without the band form of the residual");
-  if (batch < 1 || batch > p->g.max_batch) return fail(OI_ERR_INVALID, "batch outside [1, max_batch]");
-  if (nkern < 1 || nkern > p->g.max_kernels)
-    return fail(OI_ERR_INVALID, "socs_band_residual_adjoint: nkern outside [1, max_kernels]");
-  hipStream_t st = (hipStream_t)stream;
+// The band form of one gradient step for real masks `field` (logits when
+// `logits`): spectrum, coherent sum up to the image band, cotangent spectrum
+// and loss from the band, adjoint.  With logits the adjoint's last pass
+// writes the gradient with respect to the logits and its norm's partial sums
+// (in p->part, which the loss sums have left by then: same stream).
+static int band_step_impl(oi_plan* p, const float* field, bool logits, int batch, const void* kern,
+                          int nkern, const void* tband, float* loss, void* work, float* grad,
+                          hipStream_t st) {
   const int chan0 = 0;
   const int slots = p->slots;
   p->slots = 0;  // stage timing covers the imaging calls only
-  int rc = spectrum_impl(p, mask, OI_FIELD_REAL, 1, &chan0, batch, p->spec, st);
+  int rc = spectrum_impl(p, field, logits ? kFieldLogits : OI_FIELD_REAL, 1, &chan0, batch,
+                         p->spec, st);
   if (!rc) rc = coherent_impl(p, p->spec, batch, kern, nkern, nullptr, nullptr, st, true);
   p->slots = slots;
   if (rc) return rc;
@@ -812,7 +809,28 @@ extern "C" int oi_socs_band_residual_adjoint(oi_plan* p, const float* mask, int 
     return rc;
   const float2* bcols = p->g.path == 2 ? p->Bbuf : nullptr;
   return launch_adjoint(ag, nullptr, p->spec, batch, static_cast<const float2*>(kern), nkern, work,
-                        grad_mask, st, bcols);
+                        grad, st, bcols, logits ? field : nullptr, logits ? p->part : nullptr);
+}
+
+static int band_step_check(const oi_plan* p, int batch, int nkern, const char* who) {
+  if (!band_residual_ok(p))
+    return fail(OI_ERR_UNSUPPORTED, std::string(who) + ": plan without the band form of the residual");
+  if (batch < 1 || batch > p->g.max_batch) return fail(OI_ERR_INVALID, "batch outside [1, max_batch]");
+  if (nkern < 1 || nkern > p->g.max_kernels)
+    return fail(OI_ERR_INVALID, std::string(who) + ": nkern outside [1, max_kernels]");
+  return OI_OK;
+}
+
+extern "C" int oi_socs_band_residual_adjoint(oi_plan* p, const float* mask, int batch,
+                                             const void* kern, int nkern, const void* tband,
+                                             float* loss, void* work, float* grad_mask,
+                                             void* stream) {
+  if (!p || !mask || !kern || !tband || !work || !grad_mask)
+    return fail(OI_ERR_INVALID, "socs_band_residual_adjoint: null argument");
+  int rc = band_step_check(p, batch, nkern, "socs_band_residual_adjoint");
+  if (rc) return rc;
+  return band_step_impl(p, mask, false, batch, kern, nkern, tband, loss, work, grad_mask,
+                        (hipStream_t)stream);
 }
 
 extern "C" int oi_ilt_band_step(oi_plan* p, float* theta, int batch, const void* kern, int nkern,
@@ -821,35 +839,12 @@ extern "C" int oi_ilt_band_step(oi_plan* p, float* theta, int batch, const void*
   if (!p || !theta || !kern || !tband || !work || !grad_theta)
     return fail(OI_ERR_INVALID, "ilt_band_step: null argument");
   if (grad_theta == theta) return fail(OI_ERR_INVALID, "ilt_band_step: grad_theta must not be theta");
-  if (!band_residual_ok(p) || !adjoint_chain_ok(adj_geom(p)))
+  int rc = band_step_check(p, batch, nkern, "ilt_band_step");
+  if (rc) return rc;
+  if (!adjoint_chain_ok(adj_geom(p)))
     return fail(OI_ERR_UNSUPPORTED, "ilt_band_step: plan without the band form of the residual");
-  if (batch < 1 || batch > p->g.max_batch) return fail(OI_ERR_INVALID, "batch outside [1, max_batch]");
-  if (nkern < 1 || nkern > p->g.max_kernels)
-    return fail(OI_ERR_INVALID, "ilt_band_step: nkern outside [1, max_kernels]");
   hipStream_t st = (hipStream_t)stream;
-  const int chan0 = 0;
-  const int slots = p->slots;
-  p->slots = 0;  // stage timing covers the imaging calls only
-  // spectrum of m = logistic(theta), coherent sum up to the image band
-  int rc = spectrum_impl(p, theta, kFieldLogits, 1, &chan0, batch, p->spec, st);
-  if (!rc) rc = coherent_impl(p, p->spec, batch, kern, nkern, nullptr, nullptr, st, true);
-  p->slots = slots;
-  if (rc) return rc;
-  const AdjGeom ag = adj_geom(p);
-  const float2* band = static_cast<const float2*>(tband);
-  const float* tconst = reinterpret_cast<const float*>(static_cast<const char*>(tband) +
-                                                       band_const_offset(p, batch));
-  rc = ilt_band_cotangent(p->Ihat, p->g.qr, p->g.qc, band, tconst,
-                          (float)p->g.ny * (float)p->g.nx, batch,
-                          adjoint_cotangent_spectrum(ag, batch, work), p->part, st);
-  if (rc) return rc;
-  if (loss && (rc = ilt_sum(p->part, batch * ilt_band_blocks(p->g.qr, p->g.qc), loss, st)))
-    return rc;
-  // the loss sums are consumed (same stream): p->part takes the gradient's
-  // norm sums of the adjoint's last pass
-  const float2* bcols = p->g.path == 2 ? p->Bbuf : nullptr;
-  rc = launch_adjoint(ag, nullptr, p->spec, batch, static_cast<const float2*>(kern), nkern, work,
-                      grad_theta, st, bcols, theta, p->part);
+  rc = band_step_impl(p, theta, true, batch, kern, nkern, tband, loss, work, grad_theta, st);
   if (rc) return rc;
   return ilt_update_theta(theta, grad_theta, lr, batch, (size_t)p->g.ny * p->g.nx, p->part,
                           fp_urows_blocks(p->g.ny, p->g.nx, batch), st);

@@ -220,11 +220,15 @@ def serving_leg(count, rate, fractions=(0.25, 0.5, 0.75, 0.9)):
     codes = serving.pinned_codes(range(count), sizes, workloads.serving_mask_codes)
     t_gen = time.perf_counter() - t
     masks = serving.pinned_masks(range(nb), sizes[:nb], workloads.serving_mask)   # float32 form
+    bits = serving.pinned_bits(range(nb), sizes[:nb], workloads.serving_mask_codes)  # 1 bit/pixel
     ex = serving.GpuExecutor({c.n: c for c in classes}, streams=4)
     budget = 8e9
 
     def coded(i, at):
         return serving.Request(i, codes[i][0], at, levels=codes[i][1])
+
+    def packed(i, at):
+        return serving.Request(i, bits[i][0], at, levels=bits[i][1], packed=True)
 
     # warm-up: the capacity run below once, so that every (class, stream,
     # batch size) has its device buffers in torch's caching allocator
@@ -232,6 +236,8 @@ def serving_leg(count, rate, fractions=(0.25, 0.5, 0.75, 0.9)):
         [coded(i, 0.0) for i in range(nb)])
     serving.Server(classes, ex, hbm_budget=budget, keep_images=False).run(
         [serving.Request(i, masks[i], 0.0) for i in range(nb)])
+    serving.Server(classes, ex, hbm_budget=budget, keep_images=False).run(
+        [packed(i, 0.0) for i in range(nb)])
     torch.cuda.synchronize()
     # capacity: the first 512 clips all present at t = 0, coded and as float32;
     # the coded run is repeated and its median taken (single runs spread by
@@ -244,6 +250,12 @@ def serving_leg(count, rate, fractions=(0.25, 0.5, 0.75, 0.9)):
     cap = caps[len(caps) // 2]
     srv = serving.Server(classes, ex, hbm_budget=budget, keep_images=False)
     cap_f32 = serving.latency_stats(srv.run([serving.Request(i, masks[i], 0.0) for i in range(nb)]))
+    caps_bits = []
+    for _ in range(3):
+        srv = serving.Server(classes, ex, hbm_budget=budget, keep_images=False)
+        caps_bits.append(serving.latency_stats(srv.run([packed(i, 0.0) for i in range(nb)])))
+    caps_bits.sort(key=lambda c: c["images_per_s"])
+    cap_bits = caps_bits[len(caps_bits) // 2]
 
     def offered(r):
         srv = serving.Server(classes, ex, hbm_budget=budget, keep_images=False)
@@ -279,6 +291,8 @@ def serving_leg(count, rate, fractions=(0.25, 0.5, 0.75, 0.9)):
            "capacity_runs_images_per_s": [round(c["images_per_s"], 1) for c in caps],
            "capacity_mean_batch": round(cap["mean_batch"], 2),
            "capacity_float32_transfer_images_per_s": round(cap_f32["images_per_s"], 1),
+           "capacity_bit_packed_transfer_images_per_s": round(cap_bits["images_per_s"], 1),
+           "capacity_bit_packed_runs_images_per_s": [round(c["images_per_s"], 1) for c in caps_bits],
            "load_curve": curve,
            "kernel_build_s": round(t_ks, 2), "mask_generation_s": round(t_gen, 2)}
     out.update(main)

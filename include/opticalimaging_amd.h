@@ -213,6 +213,14 @@ int oi_kernel_dense(const void* win, int nk, int k0r, int wr, int k0c, int wc,
  * bytes this way.                                                           */
 int oi_mask_decode_u8(const void* codes, size_t count, const float* levels, int nlevels,
                       float* out, void* stream);
+/* Two-level masks sent as one bit per pixel (serving): out[i] = levels[b_i]
+ * for i < count, with b_i = bit 7 - (i mod 8) of byte i / 8 of `bits` (the
+ * order of numpy.packbits; a clip packed row by row has rows of n / 8 bytes
+ * when n is a multiple of 8).  levels is a host array of 2 floats; bits is a
+ * device pointer aligned to 4 bytes, out one aligned to 16.  No counterpart
+ * in the reference; a binary clip crosses PCIe in 1/32 of its float32 bytes. */
+int oi_mask_decode_bits(const void* bits, size_t count, const float* levels, float* out,
+                        void* stream);
 
 /* ----------------------------------------------------------- pupil / TCC --
  * Optics of one focus plane (Projection.py:5-10, Source.py:18).            */

@@ -80,6 +80,8 @@ SIGNATURES = {
                                 c_void_p, c_void_p]),
     "oi_mask_decode_u8": (c_int, [c_void_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_float), c_int,
                                   c_void_p, c_void_p]),
+    "oi_mask_decode_bits": (c_int, [c_void_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_float),
+                                    c_void_p, c_void_p]),
     "oi_pupil_kernels": (c_int, [ctypes.POINTER(Optics), c_void_p, c_void_p, c_void_p, c_void_p,
                                  c_void_p, c_int, c_int, c_int, c_int, c_int, c_void_p, c_void_p]),
     "oi_tcc_work_bytes": (c_size_t, [c_int, c_int, c_int, c_int]),

@@ -604,7 +604,48 @@ __global__ __launch_bounds__(256) void k_decode_u8(const uint8_t* __restrict__ c
     for (size_t i = n16 * 16 + threadIdx.x; i < n; i += blockDim.x) out[i] = lut[codes[i]];
   }
 }
+
+// One bit per pixel: each thread expands 4 bytes (32 pixels, bit 7 of a byte
+// first) into eight 16-byte stores.
+__global__ __launch_bounds__(256) void k_decode_bits(const uint8_t* __restrict__ bits, size_t n,
+                                                     float v0, float v1, float* __restrict__ out) {
+  const size_t n32 = n / 32;
+  const uint32_t* b4 = reinterpret_cast<const uint32_t*>(bits);
+  float4* o4 = reinterpret_cast<float4*>(out);
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n32; i += stride) {
+    const uint32_t w = b4[i];
+#pragma unroll
+    for (int byte = 0; byte < 4; ++byte) {
+      const uint32_t c = (w >> (8 * byte)) & 0xffu;     // little-endian words: byte order kept
+      o4[8 * i + 2 * byte] = make_float4((c & 0x80u) ? v1 : v0, (c & 0x40u) ? v1 : v0,
+                                         (c & 0x20u) ? v1 : v0, (c & 0x10u) ? v1 : v0);
+      o4[8 * i + 2 * byte + 1] = make_float4((c & 0x08u) ? v1 : v0, (c & 0x04u) ? v1 : v0,
+                                             (c & 0x02u) ? v1 : v0, (c & 0x01u) ? v1 : v0);
+    }
+  }
+  if (blockIdx.x == 0) {
+    for (size_t i = n32 * 32 + threadIdx.x; i < n; i += blockDim.x)
+      out[i] = ((bits[i >> 3] >> (7 - (i & 7))) & 1u) ? v1 : v0;
+  }
+}
 }  // namespace
+
+extern "C" int oi_mask_decode_bits(const void* bits, size_t count, const float* levels, float* out,
+                                   void* stream) {
+  if (!bits || !out || !levels) return fail(OI_ERR_INVALID, "mask_decode_bits: null pointer");
+  if ((reinterpret_cast<uintptr_t>(bits) & 3) || (reinterpret_cast<uintptr_t>(out) & 15))
+    return fail(OI_ERR_INVALID, "mask_decode_bits: bits must be 4-byte and out 16-byte aligned");
+  if (count == 0) return OI_OK;
+  const size_t n32 = count / 32;
+  size_t blocks = (n32 + 255) / 256;
+  if (blocks < 1) blocks = 1;
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(k_decode_bits, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,
+                     static_cast<const uint8_t*>(bits), count, levels[0], levels[1], out);
+  OI_LAUNCH_CHECK("mask_decode_bits");
+  return OI_OK;
+}
 
 extern "C" int oi_mask_decode_u8(const void* codes, size_t count, const float* levels, int nlevels,
                                  float* out, void* stream) {

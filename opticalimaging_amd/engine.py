@@ -717,6 +717,27 @@ def socs_adjoint(masks: torch.Tensor, grad_image: torch.Tensor, ks: KernelSet,
     return out
 
 
+def decode_mask_bits(bits: torch.Tensor, levels, n: int, out: torch.Tensor | None = None) -> torch.Tensor:
+    """float32 masks of `n` pixels from one bit per pixel on the device
+    (oi_mask_decode_bits): pixel i is levels[1] when bit 7 - i mod 8 of byte
+    i / 8 is set, else levels[0] (numpy.packbits order).  `bits` is a uint8
+    device tensor of at least ceil(n / 8) bytes."""
+    if bits.dtype != torch.uint8 or not bits.is_cuda or not bits.is_contiguous():
+        raise ValueError("mask bits must be a contiguous uint8 tensor on the device")
+    if bits.numel() * 8 < n:
+        raise ValueError(f"{bits.numel()} bytes hold fewer than {n} pixels")
+    lv = [float(v) for v in levels]
+    if len(lv) != 2:
+        raise ValueError("bit-packed masks take exactly two levels")
+    if out is None:
+        out = torch.empty(n, dtype=torch.float32, device=bits.device)
+    if out.dtype != torch.float32 or out.numel() != n or not out.is_contiguous():
+        raise ValueError("out must be a contiguous float32 tensor of n values")
+    arr = (ctypes.c_float * 2)(*lv)
+    nat.check(nat.load().oi_mask_decode_bits(nat.ptr(bits), n, arr, nat.ptr(out), nat.stream()))
+    return out
+
+
 def decode_mask_codes(codes: torch.Tensor, levels, out: torch.Tensor | None = None) -> torch.Tensor:
     """float32 masks levels[codes] from uint8 codes on the device
     (oi_mask_decode_u8): masks with at most 256 distinct values cross the host

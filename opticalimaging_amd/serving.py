@@ -5,8 +5,9 @@ with its own device-resident kernel set).  One host thread runs the loop:
 
   * arrivals join a FIFO; a request is *admitted* when its device bytes (mask
     in + image out, float32, plus one byte per pixel when the mask arrives as
-    codes into a table of levels) fit in what the HBM budget leaves after the
-    resident kernel sets and plan scratch;
+    codes into a table of levels, or one bit per pixel when a two-level clip
+    arrives packed) fit in what the HBM budget leaves after the resident
+    kernel sets and plan scratch;
   * admitted requests wait per size class; whenever one of S execution slots
     (a HIP stream with its own plans) is free, the class holding the oldest
     admitted request launches a batch of up to its max_batch requests --
@@ -35,16 +36,18 @@ import numpy as np
 import torch
 
 __all__ = ["Request", "Completed", "SizeClass", "Server", "GpuExecutor", "latency_stats",
-           "size_classes", "pinned_masks", "pinned_codes"]
+           "size_classes", "pinned_masks", "pinned_codes", "pinned_bits"]
 
 
 @dataclass
 class Request:
     rid: int
     mask: object            # (n, n) float32 host array (numpy or pinned torch), or uint8
-                            # codes into `levels`
+                            # codes into `levels`, or (packed) the clip's bits
     arrival: float = 0.0    # seconds from the start of the run
     levels: tuple | None = None   # pixel value of each code when `mask` holds uint8 codes
+    packed: bool = False    # mask is (n, n / 8) uint8, numpy.packbits of the 0/1 codes of
+                            # each row, and levels holds the two pixel values
 
     @property
     def n(self) -> int:
@@ -53,6 +56,11 @@ class Request:
     @property
     def coded(self) -> bool:
         return self.levels is not None
+
+    def code_bytes(self) -> int:
+        """Bytes of the codes on the device (0 for a float32 mask)."""
+        n = self.n
+        return 0 if not self.coded else (n * n // 8 if self.packed else n * n)
 
 
 @dataclass
@@ -88,9 +96,9 @@ class SizeClass:
         return 2 * 4 * self.n * self.n
 
     def bytes_for(self, req: "Request") -> int:
-        """Device bytes of one request: float32 mask and image, plus the byte
-        codes when the mask arrives coded."""
-        return self.request_bytes + (self.n * self.n if req.coded else 0)
+        """Device bytes of one request: float32 mask and image, plus the codes
+        (a byte or a bit per pixel) when the mask arrives coded."""
+        return self.request_bytes + req.code_bytes()
 
 
 DEFAULT_MAX_BATCH = {128: 64, 256: 64, 512: 32, 1024: 16, 2048: 4}
@@ -136,6 +144,27 @@ def pinned_codes(rids, sizes, make):
     return out
 
 
+def pinned_bits(rids, sizes, make):
+    """Two-level request masks packed one bit per pixel in one pinned host
+    allocation: make(rid, n) -> (0/1 uint8 codes (n, n), levels); returns a
+    list of (pinned (n, n / 8) bits, levels), rows packed by numpy.packbits."""
+    sizes = [int(n) for n in sizes]
+    if any(n % 8 for n in sizes):
+        raise ValueError("bit-packed clips need n divisible by 8")
+    pool = torch.empty(sum(n * n // 8 for n in sizes), dtype=torch.uint8, pin_memory=True)
+    out = []
+    off = 0
+    for rid, n in zip(rids, sizes):
+        t = pool[off:off + n * n // 8].view(n, n // 8)
+        codes, levels = make(rid, n)
+        if len(levels) != 2:
+            raise ValueError("bit-packed clips take two levels")
+        t.numpy()[...] = np.packbits(np.asarray(codes, dtype=np.uint8), axis=1)
+        out.append((t, tuple(float(v) for v in levels)))
+        off += n * n // 8
+    return out
+
+
 def pinned_masks(rids, sizes, make):
     """Request masks generated straight into one pinned host allocation:
     returns a list of (n, n) float32 pinned tensors (make(rid, n) -> array)."""
@@ -165,6 +194,7 @@ class GpuExecutor:
         self.streams = [torch.cuda.Stream() for _ in range(streams)]
         self.plans = {}
         self.inputs = {}
+        self.bits = {}
         self.resident = 0
         for c in classes.values():
             ks = c.kernels
@@ -182,6 +212,10 @@ class GpuExecutor:
                 self.inputs[(c.n, si)] = (torch.empty(shape, dtype=torch.float32, device="cuda"),
                                           torch.empty(shape, dtype=torch.uint8, device="cuda"))
                 self.resident += 5 * c.max_batch * c.n * c.n
+                if c.n % 8 == 0:
+                    self.bits[(c.n, si)] = torch.empty((c.max_batch, c.n * c.n // 8),
+                                                       dtype=torch.uint8, device="cuda")
+                    self.resident += c.max_batch * c.n * c.n // 8
 
     @property
     def slots(self) -> int:
@@ -197,15 +231,25 @@ class GpuExecutor:
         with torch.cuda.stream(s):
             dev_in = fin[:b]
             codes = cin[:b]
-            coded = [i for i, r in enumerate(reqs) if r.coded]
+            bits = self.bits.get((n, slot))
+            coded = [i for i, r in enumerate(reqs) if r.coded and not r.packed]
+            packed = [i for i, r in enumerate(reqs) if r.packed]
             for i, m in enumerate(srcs):      # pinned host memory: asynchronous copies on s
-                dst = codes[i] if reqs[i].coded else dev_in[i]
+                r = reqs[i]
+                dst = bits[i].view(n, n // 8) if r.packed else (codes[i] if r.coded else dev_in[i])
                 dst.copy_(m, non_blocking=m.is_pinned())
             if coded and len(coded) == b and all(r.levels == reqs[0].levels for r in reqs):
                 self.engine.decode_mask_codes(codes, reqs[0].levels, out=dev_in)
             else:
                 for i in coded:
                     self.engine.decode_mask_codes(codes[i], reqs[i].levels, out=dev_in[i])
+            if packed and len(packed) == b and all(r.levels == reqs[0].levels for r in reqs):
+                self.engine.decode_mask_bits(bits[:b].reshape(-1), reqs[0].levels, b * n * n,
+                                             out=dev_in.view(-1))
+            else:
+                for i in packed:
+                    self.engine.decode_mask_bits(bits[i], reqs[i].levels, n * n,
+                                                 out=dev_in[i].view(-1))
             out = self.engine.socs_image(dev_in, cls.kernels, out=torch.empty_like(dev_in),
                                          plan=self.plans[(n, slot)])
             ev = torch.cuda.Event()
@@ -277,18 +321,25 @@ class Server:
         """Why request r cannot be served, or None."""
         mask = r.mask
         shape = tuple(getattr(mask, "shape", ()))
-        if len(shape) != 2 or shape[0] != shape[1]:
+        dt = str(getattr(mask, "dtype", ""))
+        if r.packed:
+            if r.levels is None or len(r.levels) != 2:
+                return "a bit-packed mask takes a table of two levels"
+            if len(shape) != 2 or shape[0] % 8 or shape[1] * 8 != shape[0]:
+                return f"a bit-packed mask is (n, n / 8) with n divisible by 8, got shape {shape}"
+            if "uint8" not in dt:
+                return f"bit-packed masks are uint8, got {dt}"
+        elif len(shape) != 2 or shape[0] != shape[1]:
             return f"mask must be a square (n, n) array, got shape {shape}"
         c = self.classes.get(int(shape[0]))
         if c is None:
             return f"no size class for {shape[0]}^2 requests"
-        dt = str(getattr(mask, "dtype", ""))
-        if r.coded:
+        if r.coded and not r.packed:
             if "uint8" not in dt:
                 return f"coded masks are uint8, got {dt}"
             if not 1 <= len(r.levels) <= 256:
                 return "a level table holds 1 to 256 values"
-        elif "float32" not in dt:
+        elif not r.coded and "float32" not in dt:
             return f"masks are float32 (or uint8 codes with levels), got {dt}"
         if c.bytes_for(r) > self.room:
             return f"request {r.rid} ({c.bytes_for(r)} B) does not fit in the HBM budget"

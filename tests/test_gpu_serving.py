@@ -90,6 +90,62 @@ def test_coded_requests_equal_float_requests(gpu):
     assert worst == 0.0
 
 
+def test_mask_bits_decode_kernel(gpu):
+    """oi_mask_decode_bits: levels[bit] exactly in numpy.packbits order, any
+    pixel count (32-pixel body and tail); misaligned pointers are refused."""
+    from opticalimaging_amd import engine
+    rng = np.random.default_rng(2)
+    levels = (0.125, -2.5)
+    for count in (1, 7, 8, 31, 32, 33, 1000, 4096 * 32 * 256 + 13):
+        b = rng.integers(0, 2, count).astype(np.uint8)
+        packed = np.packbits(b)
+        buf = torch.zeros(packed.size + 4, dtype=torch.uint8)
+        buf[:packed.size] = torch.as_tensor(packed)
+        out = engine.decode_mask_bits(buf.cuda(), levels, count)
+        want = torch.tensor(levels, dtype=torch.float32)[torch.as_tensor(b).long()]
+        assert torch.equal(out.cpu(), want), count
+    buf = torch.zeros(64, dtype=torch.uint8, device="cuda")
+    with pytest.raises(ValueError):
+        engine.decode_mask_bits(buf[1:33], levels, 64)
+    with pytest.raises(ValueError):
+        engine.decode_mask_bits(buf, (0.0, 1.0, 2.0), 64)
+
+
+def test_packed_requests_equal_float_requests(gpu):
+    """The same clips sent one bit per pixel, as byte codes and as float32
+    (mixed in one stream) give bit-identical images."""
+    from opticalimaging_amd import serving, workloads
+    src, proj, num = workloads.ilt_optics()
+    classes = serving.size_classes((128, 256), src, proj, num, max_batch={128: 4, 256: 4})
+    t, sizes = workloads.serving_arrivals(24, rate=50000.0, sizes=(128, 256), seed=11)
+    masks = serving.pinned_masks(range(24), sizes, workloads.serving_mask)
+    codes = serving.pinned_codes(range(24), sizes, workloads.serving_mask_codes)
+    bits = serving.pinned_bits(range(24), sizes, workloads.serving_mask_codes)
+    ex = serving.GpuExecutor({c.n: c for c in classes}, streams=2)
+    budget = ex.resident + 64 * 2**20
+    a = serving.Server(classes, ex, hbm_budget=budget).run(
+        [serving.Request(i, masks[i], float(t[i])) for i in range(24)])
+
+    def mixed(i):
+        if i % 3 == 0:
+            return serving.Request(i, bits[i][0], float(t[i]), levels=bits[i][1], packed=True)
+        if i % 3 == 1:
+            return serving.Request(i, codes[i][0], float(t[i]), levels=codes[i][1])
+        return serving.Request(i, masks[i], float(t[i]))
+    b = serving.Server(classes, ex, hbm_budget=budget).run([mixed(i) for i in range(24)])
+    c = serving.Server(classes, ex, hbm_budget=budget).run(
+        [serving.Request(i, bits[i][0], float(t[i]), levels=bits[i][1], packed=True)
+         for i in range(24)])
+    torch.cuda.synchronize()
+    assert all(x.ok for x in a + b + c)
+    ia = {x.rid: x.image.cpu() for x in a}
+    for run in (b, c):
+        ib = {x.rid: x.image.cpu() for x in run}
+        worst = max(float((ia[i] - ib[i]).abs().max()) for i in range(24))
+        print("max |float - packed or mixed| over 24 requests:", worst)
+        assert worst == 0.0
+
+
 def test_kernel_set_warm_start(gpu, tmp_path):
     """serving.size_classes with a cache directory: the first call builds and
     saves each size's kernel set, the second loads them (checkpoint) and the

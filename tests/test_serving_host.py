@@ -249,3 +249,32 @@ def test_run_cancel_at():
     done = srv.run(reqs, clock=tick, idle_sleep=0.0, cancel_at={2: 0.0})
     by = {c.rid: c.status for c in done}
     assert by == {0: "ok", 1: "ok", 2: "cancelled"}
+
+
+def test_packed_requests_are_validated_and_sized():
+    """Bit-packed clips: (n, n / 8) uint8 with two levels; their device bytes
+    count one bit per pixel of codes."""
+    clock = [0.0]
+    ex = SimExecutor(clock)
+    classes = _classes()
+    srv = Server(classes, ex, hbm_budget=1e9)
+    good = Request(0, np.zeros((256, 32), np.uint8), 0.0, levels=(0.0, 1.0), packed=True)
+    assert srv.problem(good) is None
+    c = {k.n: k for k in classes}[256]
+    assert c.bytes_for(good) == 2 * 4 * 256 * 256 + 256 * 256 // 8
+    bad = [
+        Request(1, np.zeros((256, 256), np.uint8), 0.0, levels=(0.0, 1.0), packed=True),   # unpacked shape
+        Request(2, np.zeros((256, 32), np.uint8), 0.0, levels=(0.0, 0.5, 1.0), packed=True),  # three levels
+        Request(3, np.zeros((256, 32), np.float32), 0.0, levels=(0.0, 1.0), packed=True),  # dtype
+        Request(4, np.zeros((260, 32), np.uint8), 0.0, levels=(0.0, 1.0), packed=True),   # n not /8
+        Request(5, np.zeros((256, 32), np.uint8), 0.0, packed=True),                       # no levels
+        Request(6, np.zeros((96, 12), np.uint8), 0.0, levels=(0.0, 1.0), packed=True),    # no class
+    ]
+    for r in bad:
+        assert srv.problem(r) is not None, r.rid
+    # a mixed stream of packed, coded and float32 clips completes
+    reqs = [Request(10, np.zeros((256, 32), np.uint8), 0.0, levels=(0.0, 1.0), packed=True),
+            Request(11, np.zeros((512, 512), np.uint8), 0.0, levels=(0.0, 1.0)),
+            Request(12, np.zeros((1024, 1024), np.float32), 0.0)]
+    _drive(srv, ex, reqs)
+    assert sorted(d.rid for d in srv.completed if d.ok) == [10, 11, 12]

@@ -751,67 +751,18 @@ __global__ __launch_bounds__(kBlock, 4) void k_fp_brows(const float2* __restrict
 // ------------------------------------------------------- band extraction
 // Ihat[img][q][i] = scale * sum_t (sum_g R[img][g][q][t]) e^{+2 pi i qr t / L},
 // qr = i - qr_max, i in [0, 2 qr_max + 1).
-// Small blocks (kBandBlock threads): the band has few orders per image, so
-// several blocks per image keep the grid wide.
-constexpr int kBandBlock = 64;
-
-template <class PL>
-__global__ __launch_bounds__(kBandBlock) void k_fp_band(const float2* __restrict__ R, int groups,
-                                                        int qc, int qrm, float scale,
-                                                        const float2* __restrict__ twt,
-                                                        float2* __restrict__ Ihat) {
-  constexpr int L = PL::N, TL = PL::T, VL = PL::V, G = kBandBlock / TL;
-  static_assert(G >= 1, "band block holds one transform");
-  extern __shared__ float2 lds[];
-  const int g = threadIdx.x / TL, t = threadIdx.x - g * TL;
-  const bool live = g < G;
-  const int gg = live ? g : 0;
-  const int img = blockIdx.y;
-  const int q = blockIdx.x * G + gg;
-  const int nq = qc + 1;
-  const bool on = live && q < nq;
-  float2 v[VL];
-#pragma unroll
-  for (int m = 0; m < VL; ++m) {
-    float2 sum = make_float2(0.f, 0.f);
-    if (on) {
-      const float2* src = R + ((size_t)img * groups * nq + q) * L + (t + TL * m);
-#pragma unroll 4
-      for (int k = 0; k < groups; ++k) {
-        const float2 x = src[(size_t)k * nq * L];
-        sum.x += x.x;
-        sum.y += x.y;
-      }
-    }
-    v[m] = sum;
-  }
-  PL::template run_live_tab<true>(v, lds + g * PL::LDS, t, twt, true);
-  if (!on) return;
-  const int nb = 2 * qrm + 1;
-  float2* dst = Ihat + ((size_t)img * nq + q) * nb;
-#pragma unroll
-  for (int m = 0; m < VL; ++m) {
-    const int x = t + TL * m;
-    const int qr = x <= qrm ? x : x - L;
-    if (qr >= -qrm && qr <= qrm) dst[qr + qrm] = make_float2(v[m].x * scale, v[m].y * scale);
-  }
-}
-
-// The same band for many kernel groups (small batches, whose coherent sum
-// splits the kernels over up to 64 groups): one order q per workgroup of
-// kBandWide threads.  The group sum is spread over the threads (S = kBandWide /
-// L slices of the groups, each slice summed in order, then the slices in
-// order), so it no longer runs as `groups` dependent loads per thread of a
-// handful of workgroups.
+// One order q per workgroup of kBandWide threads.  The sum over the kernel
+// groups (up to 64 for a batch of one image) is spread over the threads:
+// S = kBandWide / L slices of the groups, each slice summed in order, then the
+// slices in order, so that it never runs as `groups` dependent loads in a
+// handful of threads; then one group of threads transforms the column.
 constexpr int kBandWide = 256;
-constexpr int kBandWideMinGroups = 9;   // fewer groups: k_fp_band
 
 template <class PL>
-__global__ __launch_bounds__(kBandWide) void k_fp_band_wide(const float2* __restrict__ R,
-                                                            int groups, int qc, int qrm,
-                                                            float scale,
-                                                            const float2* __restrict__ twt,
-                                                            float2* __restrict__ Ihat) {
+__global__ __launch_bounds__(kBandWide) void k_fp_band(const float2* __restrict__ R, int groups,
+                                                       int qc, int qrm, float scale,
+                                                       const float2* __restrict__ twt,
+                                                       float2* __restrict__ Ihat) {
   constexpr int L = PL::N, TL = PL::T, VL = PL::V;
   constexpr int S = kBandWide / L > 0 ? kBandWide / L : 1;   // group slices
   static_assert(TL <= kBandWide, "one transform group");
@@ -1259,25 +1210,13 @@ static int run_coherent(const FpArgs& a, const FpCoh& c, hipStream_t st) {
 
 template <class PL>
 static int run_band(const FpArgs& a, int groups, hipStream_t st) {
-  if (groups >= kBandWideMinGroups) {
-    constexpr int L = PL::N, S = kBandWide / L > 0 ? kBandWide / L : 1;
-    const size_t lds = ((size_t)S * L + L + PL::LDS) * sizeof(float2);
-    int rc = allow(k_fp_band_wide<PL>, lds);
-    if (rc) return rc;
-    const float scale = 1.0f / ((float)PL::N * (float)PL::N);
-    hipLaunchKernelGGL(k_fp_band_wide<PL>, dim3(a.qc + 1, a.batch), dim3(kBandWide), lds, st, a.R,
-                       groups, a.qc, a.qr, scale, a.tw_l, a.Ihat);
-    OI_LAUNCH_CHECK("fp_band_wide");
-    return OI_OK;
-  }
-  constexpr int G = kBandBlock / PL::T;
-  const size_t lds = (size_t)((kBandBlock + PL::T - 1) / PL::T) * PL::LDS * sizeof(float2);
+  constexpr int L = PL::N, S = kBandWide / L > 0 ? kBandWide / L : 1;
+  const size_t lds = ((size_t)S * L + L + PL::LDS) * sizeof(float2);
   int rc = allow(k_fp_band<PL>, lds);
   if (rc) return rc;
-  dim3 grid((a.qc + 1 + G - 1) / G, a.batch);
   const float scale = 1.0f / ((float)PL::N * (float)PL::N);
-  hipLaunchKernelGGL(k_fp_band<PL>, grid, dim3(kBandBlock), lds, st, a.R, groups, a.qc, a.qr, scale,
-                     a.tw_l, a.Ihat);
+  hipLaunchKernelGGL(k_fp_band<PL>, dim3(a.qc + 1, a.batch), dim3(kBandWide), lds, st, a.R, groups,
+                     a.qc, a.qr, scale, a.tw_l, a.Ihat);
   OI_LAUNCH_CHECK("fp_band");
   return OI_OK;
 }

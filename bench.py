@@ -396,7 +396,9 @@ def ilt_leg(steps):
     torch.cuda.synchronize()
     dt = time.perf_counter() - t
     lo = losses.cpu().numpy()
-    b_alg = 16 * 1024 * 1024 * 12            # theta in, theta and mask out (float32)
+    # the full-size float32 arrays a step streams: theta read three times, the
+    # theta gradient written and read, theta written
+    b_alg = 16 * 1024 * 1024 * 24
     return {"steps": steps, "batch": 16, "image": [1024, 1024], "kernels": ks.K,
             "paired_kernels": ks.for_real_masks().K,
             "ms_per_step": round(dt / steps * 1e3, 3),

@@ -303,6 +303,50 @@ def test_ilt_band_step_matches_separate_calls(gpu, case):
         assert lerr < 1e-6 and peak > 0 and gerr <= 1e-5 and terr < 5e-5
 
 
+def test_band_step_non_square(gpu):
+    """The band form on a 128 x 256 grid (row and column bands of different
+    widths, compile-time-size path): oi_socs_band_residual_adjoint against
+    oi_socs_residual_adjoint (loss to 1e-5 relative, gradient to 2e-6 of its
+    peak), and the loss and theta gradient of oi_ilt_band_step against the
+    same two calls on sigmoid(theta)."""
+    from opticalimaging_amd import engine
+    ny, nx, pxx, pyy = 128, 256, 2000.0, 1000.0
+    src, proj, num = annular(), projector(), numerics(order=20)
+    tcc, fg = O.tcc_matrix(src, [pyy, pxx], proj, 0.0, num)
+    kern = O.decompose_tcc_socs(tcc, fg, [nx, ny], num)
+    ks = engine.KernelSet.from_dense(torch.as_tensor(kern.astype(np.complex64)))
+    B = 2
+    rng = np.random.default_rng(17)
+    theta = torch.as_tensor(rng.standard_normal((B, ny, nx)).astype(np.float32)).cuda()
+    T = torch.as_tensor((rng.random((B, ny, nx)) < 0.3).astype(np.float32)).cuda()
+    plan = engine.plan_for(ks, 1, B)
+    info = plan.info()
+    assert info["path"] in (1, 2) and info["qr"] != info["qc"], info
+    work = engine.adjoint_work(ks, B, plan)
+    tband = engine.ilt_target_band(T, ks, work, plan)
+    assert tband is not None
+    m = torch.sigmoid(theta)
+    d1, g1 = torch.empty_like(m), torch.empty_like(m)
+    l1 = torch.zeros(1, device="cuda")
+    assert engine.socs_residual_adjoint(m, ks, T, d1, g1, work, l1, plan=plan)
+    g2 = torch.empty_like(m)
+    l2 = torch.zeros(1, device="cuda")
+    engine.socs_band_residual_adjoint(m, ks, tband, g2, work, l2, plan=plan)
+    th = theta.clone()
+    g3 = torch.empty_like(m)
+    l3 = torch.zeros(1, device="cuda")
+    engine.ilt_band_step(th, ks, tband, 0.5, g3, work, l3, plan=plan)
+    torch.cuda.synchronize()
+    peak = float(g1.abs().max())
+    gt1 = g1 * m * (1.0 - m)
+    e2 = float((g1 - g2).abs().max()) / peak
+    e3 = float((gt1 - g3).abs().max()) / float(gt1.abs().max())
+    l_err = [abs(float(x) - float(l1)) / float(l1) for x in (l2, l3)]
+    print(f"non-square {ny}x{nx} (qr {info['qr']}, qc {info['qc']}): gradient {e2:.2e}, "
+          f"theta gradient {e3:.2e}, losses {l_err}")
+    assert max(l_err) < 1e-5 and e2 <= 2e-6 and e3 <= 1e-5
+
+
 def test_band_residual_needs_the_band_form(gpu):
     """Plans without the fused image pass have no band form: the byte count
     is 0, ilt_target_band returns None and the step raises."""

@@ -160,3 +160,34 @@ def test_workloads_deterministic():
     m2, src, proj, num, period = workloads.vector_abbe_512(n=128)
     assert set(np.unique(m2)) <= {1.0, np.float32(workloads.ATT_PSM_AMPLITUDE)}
     assert num.ImageCalculationMode == "vector" and proj.Magnification == 1
+
+
+def test_band_step_and_bit_decode_reject_arguments_before_any_device_call():
+    """The argument checks of the band-form ILT entry points and of the bit
+    decode run on the host, ahead of any HIP call: a null pointer, a gradient
+    buffer that is the logits buffer itself, a misaligned bit buffer."""
+    import ctypes
+    from opticalimaging_amd import _native
+    lib = _native.load()
+    buf = (ctypes.c_float * 64)()
+    a = ctypes.cast(buf, ctypes.c_void_p)
+    b = ctypes.c_void_p(ctypes.addressof(buf) + 128)
+    null = ctypes.c_void_p()
+    assert lib.oi_ilt_target_band_bytes(null, 4) == 0
+    assert lib.oi_ilt_target_band(null, a, 1, a, a, null) == _native.OI_ERR_INVALID
+    assert lib.oi_socs_band_residual_adjoint(null, a, 1, a, 1, a, null, a, a, null) == \
+        _native.OI_ERR_INVALID
+    assert lib.oi_ilt_band_step(null, a, 1, a, 1, a, 0.1, null, a, b, null) == _native.OI_ERR_INVALID
+    # a plan pointer the call never reads: the alias check comes first
+    rc = lib.oi_ilt_band_step(a, a, 1, a, 1, a, 0.1, null, b, a, null)
+    assert rc == _native.OI_ERR_INVALID
+    with pytest.raises(ValueError, match="grad_theta must not be theta"):
+        _native.check(rc)
+    levels = (ctypes.c_float * 2)(0.0, 1.0)
+    assert lib.oi_mask_decode_bits(null, 32, levels, a, null) == _native.OI_ERR_INVALID
+    rc = lib.oi_mask_decode_bits(ctypes.c_void_p(ctypes.addressof(buf) + 2), 32, levels, a, null)
+    assert rc == _native.OI_ERR_INVALID
+    with pytest.raises(ValueError, match="4-byte"):
+        _native.check(rc)
+    al = ctypes.c_void_p((ctypes.addressof(buf) + 15) & ~15)
+    assert lib.oi_mask_decode_bits(al, 0, levels, al, null) == _native.OI_OK   # nothing to decode

@@ -2,7 +2,9 @@
 ``Scatter`` (Scatter.py:3-8).  ``CalculateSpectrum`` runs on the HIP device."""
 from __future__ import annotations
 
+import collections
 import ctypes
+import threading
 
 import torch
 
@@ -42,6 +44,38 @@ def full_spectrum(field: torch.Tensor, px: float, py: float) -> torch.Tensor:
     return hk
 
 
+# Per-grid constants of the windowed pupil image (the fftshift phase of the
+# crop and its unit-circle mask), kept on the device for the last few grids so
+# that a repeated call launches no table set-up and copies nothing to the
+# device.  The cached tensors are read, never written in place.
+_PUPIL_TABLES = collections.OrderedDict()
+_PUPIL_TABLES_SIZE = 16
+_PUPIL_LOCK = threading.Lock()
+
+
+def _pupil_tables(ny, nx, px, py, dev):
+    key = (ny, nx, float(px), float(py), dev.index)
+    with _PUPIL_LOCK:
+        hit = _PUPIL_TABLES.get(key)
+        if hit is not None:
+            _PUPIL_TABLES.move_to_end(key)
+            return hit
+    ox, oy = int(px) + 3, int(py) + 3
+    f, g = frequency_axes(nx, ny, px, py)
+    kr = torch.arange(-oy, oy, device=dev, dtype=torch.float64)
+    kc = torch.arange(-ox, ox, device=dev, dtype=torch.float64)
+    # exp(-2 pi i kappa (N//2)/N): the fftshift phase of Scatter.py:22
+    ph = torch.exp(-2j * torch.pi * (kr[:, None] * (ny // 2) / ny + kc[None, :] * (nx // 2) / nx))
+    gm, fm = torch.meshgrid(g[ny // 2 - oy: ny // 2 + oy].to(dev),
+                            f[nx // 2 - ox: nx // 2 + ox].to(dev), indexing="ij")
+    entry = (ph, (gm ** 2 + fm ** 2) > 1)
+    with _PUPIL_LOCK:
+        _PUPIL_TABLES[key] = entry
+        while len(_PUPIL_TABLES) > _PUPIL_TABLES_SIZE:
+            _PUPIL_TABLES.popitem(last=False)
+    return entry
+
+
 def pupil_image(field: torch.Tensor, px: float, py: float, hk: torch.Tensor | None = None,
                 encoding=None):
     """PupilImage of Scatter.CalculateSpectrum (Scatter.py:25-35): Hk with NaN
@@ -54,7 +88,6 @@ def pupil_image(field: torch.Tensor, px: float, py: float, hk: torch.Tensor | No
     h = torch.as_tensor(field).to(dev)
     ny, nx, cin = h.shape
     ox, oy = int(px) + 3, int(py) + 3
-    f, g = frequency_axes(nx, ny, px, py)
     inside = (ny // 2 - oy >= 0 and nx // 2 - ox >= 0 and oy <= ny - ny // 2 and ox <= nx - nx // 2)
     if inside and hk is None:
         enc = encoding if encoding is not None else engine.field_encoding(h)
@@ -66,22 +99,19 @@ def pupil_image(field: torch.Tensor, px: float, py: float, hk: torch.Tensor | No
         nat.check(nat.load().oi_spectrum_window(plan.handle, nat.ptr(tens), kind, cin_, cptr, 1,
                                                 nat.ptr(spec), nat.stream()))
         del keep
-        kr = torch.arange(-oy, oy, device=dev, dtype=torch.float64)
-        kc = torch.arange(-ox, ox, device=dev, dtype=torch.float64)
-        # Hk = px py exp(-2 pi i kappa (N//2)/N) spec (fftshift phase of Scatter.py:22)
-        ph = torch.exp(-2j * torch.pi * (kr[:, None] * (ny // 2) / ny + kc[None, :] * (nx // 2) / nx))
+        # Hk = px py exp(-2 pi i kappa (N//2)/N) spec
+        ph, outside = _pupil_tables(ny, nx, px, py, dev)
         win = torch.zeros((2 * oy, 2 * ox, cin), dtype=torch.complex64, device=dev)
         vals = (spec[0].to(torch.complex128) * ph[None] * (px * py)).to(torch.complex64)
         for i, c in enumerate(enc.source_channels):
             win[:, :, c] = vals[i]
-        gsel = g[ny // 2 - oy: ny // 2 + oy].to(dev)
-        fsel = f[nx // 2 - ox: nx // 2 + ox].to(dev)
-    else:
-        if hk is None:
-            hk = full_spectrum(h, px, py)
-        win = hk[(ny // 2) - oy:(ny // 2) + oy, (nx // 2) - ox:(nx // 2) + ox].clone()
-        gsel = g[(ny // 2) - oy:(ny // 2) + oy].to(dev)
-        fsel = f[(nx // 2) - ox:(nx // 2) + ox].to(dev)
+        return win.masked_fill_(outside[..., None], float("nan"))   # no host round trip
+    f, g = frequency_axes(nx, ny, px, py)
+    if hk is None:
+        hk = full_spectrum(h, px, py)
+    win = hk[(ny // 2) - oy:(ny // 2) + oy, (nx // 2) - ox:(nx // 2) + ox].clone()
+    gsel = g[(ny // 2) - oy:(ny // 2) + oy].to(dev)
+    fsel = f[(nx // 2) - ox:(nx // 2) + ox].to(dev)
     gm, fm = torch.meshgrid(gsel, fsel, indexing="ij")
     outside = (gm ** 2 + fm ** 2) > 1
     return win.masked_fill_(outside[..., None], float("nan"))   # no host round trip

@@ -311,9 +311,10 @@ def abbe_configs_leg():
         sc = Scatter()
         sc.Period_X = sc.Period_Y = period
         sc.ScatterField = workloads.field_from_mask(mask).cuda()
-        CalculateAbbeImage(src, sc, proj, num)            # warm-up (plans, twiddles)
+        for _ in range(5):                                # warm-up (plans, twiddles, caches, clocks)
+            CalculateAbbeImage(src, sc, proj, num)
         torch.cuda.synchronize()
-        reps = 5
+        reps = 50
         t = time.perf_counter()
         for _ in range(reps):
             d, _ = CalculateAbbeImage(src, sc, proj, num)

@@ -347,8 +347,11 @@ def field_encoding(field) -> FieldEncoding:
     Channels that are identically zero contribute nothing to any image
     (CalculateAerialImage.py:76-87, CalculateAerialImage_SOCS.py:5-9) and are
     skipped; a single live channel with zero imaginary part is carried as a
-    float32 mask [B, Ny, Nx]."""
+    float32 mask [B, Ny, Nx].  The channel flags are attached to a torch
+    tensor `field`, so a repeated call on the unchanged tensor reads them
+    without the device-to-host transfer (an in-place edit drops them)."""
     dev = nat.device()
+    user = field if isinstance(field, torch.Tensor) else None
     f = torch.as_tensor(field).to(device=dev)
     if f.dim() not in (3, 4):
         raise ValueError("ScatterField must be (Ny, Nx, C) or (B, Ny, Nx, C)")
@@ -360,7 +363,10 @@ def field_encoding(field) -> FieldEncoding:
     npix = g.numel() // cin
     # which channels are non-zero, and which carry an imaginary part: one pass
     # over the field (oi_field_flags) and one device-to-host transfer
-    if cin <= 16:
+    flags = attached(user, "_oi_field_flags") if user is not None else None
+    if flags is not None:
+        nonzero, imag = flags
+    elif cin <= 16:
         word = torch.zeros(1, dtype=torch.int32, device=dev)
         nat.check(nat.load().oi_field_flags(nat.ptr(g), nat.OI_FIELD_C64_CHLAST if cplx else nat.OI_FIELD_REAL,
                                             npix, cin, nat.ptr(word), nat.stream()))
@@ -372,6 +378,8 @@ def field_encoding(field) -> FieldEncoding:
         stats = torch.stack([(flat != 0).any(0), (flat.imag != 0).any(0) if cplx
                              else torch.zeros_like(flat[0], dtype=torch.bool)]).tolist()
         nonzero, imag = stats
+    if user is not None and flags is None:
+        attach(user, "_oi_field_flags", (tuple(nonzero), tuple(imag)))
     live = [c for c in range(cin) if nonzero[c]]
     if not live:
         live = [0]

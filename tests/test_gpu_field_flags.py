@@ -77,3 +77,30 @@ def test_field_encoding_choices(gpu):
     # an all-zero field keeps channel 0
     enc = engine.field_encoding(torch.zeros((8, 8, 3), device="cuda"))
     assert enc[1] == nat.OI_FIELD_REAL and enc.source_channels == [0]
+
+
+def test_field_encoding_flags_follow_in_place_edits(gpu):
+    """The channel flags attached to a field tensor are reused on an unchanged
+    tensor and dropped by an in-place edit, so an edited channel is seen."""
+    from opticalimaging_amd import _native as nat, engine
+    t = torch.zeros((16, 24, 3), dtype=torch.complex64, device="cuda")
+    t[..., 1] = 1.0
+    enc = engine.field_encoding(t)
+    assert enc[1] == nat.OI_FIELD_REAL and enc.source_channels == [1]
+    assert engine.attached(t, "_oi_field_flags") is not None
+    assert engine.field_encoding(t).source_channels == [1]          # from the attached flags
+    t[2, 3, 0] = 0.5                                                # channel 0 becomes live
+    assert engine.attached(t, "_oi_field_flags") is None
+    enc = engine.field_encoding(t)
+    assert enc[1] == nat.OI_FIELD_C64_CHLAST and enc[3] == [0, 1]
+    t[..., 1].imag.fill_(0.25)                                      # channel 1 gains an imaginary part
+    enc = engine.field_encoding(t)
+    assert enc[1] == nat.OI_FIELD_C64_CHLAST and enc[3] == [0, 1]
+    t.zero_()
+    enc = engine.field_encoding(t)
+    assert enc[1] == nat.OI_FIELD_REAL and enc.source_channels == [0]
+    # the host tensor a caller keeps carries the flags too; a numpy field has none to keep
+    h = torch.zeros((8, 8, 2))
+    h[..., 1] = 2.0
+    assert engine.field_encoding(h).source_channels == [1]
+    assert engine.attached(h, "_oi_field_flags") == ((False, True), (False, False))

@@ -6,6 +6,7 @@
 // A_m, B_m of degree <= 12 in rho, from the factorial sum of the Zernike radial
 // polynomials R_n^m; cos/sin(m theta) come from (f, g)/rho by the Chebyshev
 // recursion, so no atan2 is needed on the device.
+#include <climits>
 #include <cmath>
 #include <cstring>
 
@@ -94,7 +95,9 @@ __device__ double2 pupil_value(const PupilParams& P, double f, double g, double 
     st = g / rho;
   }
   const double nm = P.na / P.mag;
-  const double ob = pow((1.0 - nm * nm * rho2) / (1.0 - P.na * P.na * rho2), 0.25);
+  // x^(1/4) as two square roots (correctly rounded each) instead of pow's
+  // log and exp
+  const double ob = sqrt(sqrt((1.0 - nm * nm * rho2) / (1.0 - P.na * P.na * rho2)));
   const double w = zern_eval(P.z, rho, ct, st);
   // phase / pi = 2 W + 2 sqrt(n^2 - (NA/M)^2 rho^2) z / lambda
   const double ph = 2.0 * w + 2.0 * sqrt(P.n * P.n - nm * nm * rho2) * P.focus / P.lam;
@@ -128,18 +131,21 @@ __device__ void char_matrix(double f, double g, double na, double n, double2* M6
   M6[5] = make_double2(-b, 0.0);                            // Myz
 }
 
-__global__ void k_pupil(PupilParams P, const double* __restrict__ sx, const double* __restrict__ sy,
-                        const double* __restrict__ scale, const double2* __restrict__ px,
-                        const double2* __restrict__ py, int nsrc, int k0r, int wr, int k0c, int wc,
-                        float2* __restrict__ G) {
-  const size_t per = (size_t)wr * wc;
-  const size_t total = (size_t)nsrc * per;
+// Grid (window blocks, source points): workgroup (x, y) takes window entries
+// x*256 .. x*256+255 of source points y, y + gridDim.y, ..., so the point's
+// values are uniform over the workgroup and no 64-bit division is needed.
+__global__ __launch_bounds__(256) void k_pupil(PupilParams P, const double* __restrict__ sx,
+                                               const double* __restrict__ sy,
+                                               const double* __restrict__ scale,
+                                               const double2* __restrict__ px,
+                                               const double2* __restrict__ py, int nsrc, int k0r,
+                                               int wr, int k0c, int wc, float2* __restrict__ G) {
+  const int per = wr * wc;
+  const int rem = blockIdx.x * blockDim.x + threadIdx.x;
+  if (rem >= per) return;
   const int ncomp = P.vector ? 3 : 1;
-  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
-       e += (size_t)gridDim.x * blockDim.x) {
-    const int s = (int)(e / per);
-    const int rem = (int)(e - (size_t)s * per);
-    const int i = rem / wc, j = rem - i * wc;
+  const int i = rem / wc, j = rem - i * wc;
+  for (int s = blockIdx.y; s < nsrc; s += gridDim.y) {
     const double f = (double)(k0r + i) / P.pr + sx[s];
     const double g = (double)(k0c + j) / P.pc + sy[s];
     const double rho2 = f * f + g * g;
@@ -214,7 +220,10 @@ extern "C" int oi_pupil_kernels(const oi_optics* opt, const double* sx, const do
   P.pr = opt->pitch_r;
   P.pc = opt->pitch_c;
   P.vector = opt->vector ? 1 : 0;
-  hipLaunchKernelGGL(k_pupil, dim3(blocks_for((size_t)nsrc * wr * wc)), dim3(256), 0,
+  if ((size_t)wr * wc > (size_t)INT_MAX - 256)
+    return fail(OI_ERR_INVALID, "pupil_kernels: window too large");
+  const dim3 grid((unsigned)((wr * wc + 255) / 256), (unsigned)(nsrc < 65535 ? nsrc : 65535));
+  hipLaunchKernelGGL(k_pupil, grid, dim3(256), 0,
                      (hipStream_t)stream, P, sx, sy, scale, reinterpret_cast<const double2*>(px),
                      reinterpret_cast<const double2*>(py), nsrc, k0r, wr, k0c, wc,
                      static_cast<float2*>(G));

@@ -249,6 +249,18 @@ int oi_pupil_kernels(const oi_optics* opt, const double* sx, const double* sy,
                      int nsrc, int k0r, int wr, int k0c, int wc, void* G,
                      void* stream);
 
+/* The same kernels at nfoci foci in one pass (the pupil factors of
+ * CalculateAerialImage.py:64-74, which the reference re-evaluates in every
+ * pass of its focus loop, CalculateAerialImage.py:30-90): the obliquity,
+ * wavefront and polarisation of each entry are evaluated once and the
+ * defocus phase per focus.  foci: float64 [nfoci] in host memory (opt->focus
+ * is not read).  G : c64 [nfoci][ncomp][nsrc][wr][wc]; G[f] equals what
+ * oi_pupil_kernels writes with opt->focus = foci[f], bit for bit.           */
+int oi_pupil_kernels_foci(const oi_optics* opt, const double* foci, int nfoci,
+                          const double* sx, const double* sy, const double* scale,
+                          const double* px, const double* py, int nsrc, int k0r,
+                          int wr, int k0c, int wc, void* G, void* stream);
+
 /* Hermitian TCC contraction (GetTCCMatrix, CalculateTCCMatrix.py:58-76, summed
  * over components as at :48-51 and scaled as at :55):
  *   tcc[m][m'] = alpha * sum_e sum_s w[s] G[e][s][m] conj(G[e][s][m'])

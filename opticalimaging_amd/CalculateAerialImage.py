@@ -65,11 +65,16 @@ def abbe_images(field, source, scatter, projector, numerics, encoding=None) -> t
     scale = torch.sqrt(float(projector.IndexImage) * src.V / src.weight)
     foci = optics.focus_values(projector)
     out = torch.empty((len(foci), ny, nx), dtype=torch.float32, device=dev)
-    for i, z in enumerate(foci):
-        opt = optics.optics_record(source, projector, z, py, px, vector)
-        G = optics.pupil_kernels(src, opt, scale, k0r, wr, k0c, wc)
-        nat.check(lib.oi_coherent_image(plan.handle, nat.ptr(spec), 1, nat.ptr(G),
-                                        G.shape[0] * G.shape[1], nat.ptr(out[i]), nat.stream()))
+    opt = optics.optics_record(source, projector, 0.0, py, px, vector)
+    # every focus's kernels from one pass, in slices of at most 1 GiB
+    per_focus = ncomp * src.n * wr * wc * 8
+    step = max(1, min(len(foci), (1 << 30) // max(per_focus, 1)))
+    for i0 in range(0, len(foci), step):
+        G = optics.pupil_kernels_foci(src, opt, foci[i0:i0 + step], scale, k0r, wr, k0c, wc)
+        for k in range(G.shape[0]):
+            nat.check(lib.oi_coherent_image(plan.handle, nat.ptr(spec), 1, nat.ptr(G[k]),
+                                            G.shape[1] * G.shape[2], nat.ptr(out[i0 + k]),
+                                            nat.stream()))
     return out
 
 

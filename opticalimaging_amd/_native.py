@@ -84,6 +84,9 @@ SIGNATURES = {
                                     c_void_p, c_void_p]),
     "oi_pupil_kernels": (c_int, [ctypes.POINTER(Optics), c_void_p, c_void_p, c_void_p, c_void_p,
                                  c_void_p, c_int, c_int, c_int, c_int, c_int, c_void_p, c_void_p]),
+    "oi_pupil_kernels_foci": (c_int, [ctypes.POINTER(Optics), ctypes.POINTER(ctypes.c_double), c_int,
+                                      c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int,
+                                      c_int, c_int, c_int, c_void_p, c_void_p]),
     "oi_tcc_work_bytes": (c_size_t, [c_int, c_int, c_int, c_int]),
     "oi_tcc_build": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_double, c_int, c_void_p,
                              c_void_p, c_void_p]),

@@ -9,6 +9,7 @@
 #include <climits>
 #include <cmath>
 #include <cstring>
+#include <string>
 
 #include "oi_internal.h"
 
@@ -79,15 +80,24 @@ __device__ double zern_eval(const ZernPoly& z, double rho, double ct, double st)
   return w;
 }
 
+// Foci per launch of the multi-focus pupil kernel (carried in its arguments).
+constexpr int kMaxFoci = 32;
+
 struct PupilParams {
   ZernPoly z;
   double lam, na, mag, n, focus, pr, pc;
+  double foci[kMaxFoci];  // the launch's foci (foci[0] = focus for one focus)
   int vector;
 };
 
-// Scalar pupil value at normalised frequency (f, g), rho^2 = f^2 + g^2 <= 1:
-// obliquity x exp(2 pi i W) x defocus (CalculateAerialImage.py:64-67).
-__device__ double2 pupil_value(const PupilParams& P, double f, double g, double rho2) {
+// The focus-independent factors of the pupil at normalised frequency (f, g),
+// rho^2 = f^2 + g^2 <= 1: obliquity, twice the wavefront W in waves, and
+// twice the axial wavenumber sqrt(n^2 - (NA/M)^2 rho^2).
+struct PupilBase {
+  double ob, w2, kz2;
+};
+
+__device__ PupilBase pupil_base(const PupilParams& P, double f, double g, double rho2) {
   const double rho = sqrt(rho2);
   double ct = 1.0, st = 0.0;
   if (rho > 0.0) {
@@ -95,16 +105,24 @@ __device__ double2 pupil_value(const PupilParams& P, double f, double g, double 
     st = g / rho;
   }
   const double nm = P.na / P.mag;
+  PupilBase b;
   // x^(1/4) as two square roots (correctly rounded each) instead of pow's
   // log and exp
-  const double ob = sqrt(sqrt((1.0 - nm * nm * rho2) / (1.0 - P.na * P.na * rho2)));
-  const double w = zern_eval(P.z, rho, ct, st);
-  // phase / pi = 2 W + 2 sqrt(n^2 - (NA/M)^2 rho^2) z / lambda
-  const double ph = 2.0 * w + 2.0 * sqrt(P.n * P.n - nm * nm * rho2) * P.focus / P.lam;
+  b.ob = sqrt(sqrt((1.0 - nm * nm * rho2) / (1.0 - P.na * P.na * rho2)));
+  b.w2 = 2.0 * zern_eval(P.z, rho, ct, st);
+  b.kz2 = 2.0 * sqrt(P.n * P.n - nm * nm * rho2);
+  return b;
+}
+
+// obliquity x exp(2 pi i W) x defocus at focus z (CalculateAerialImage.py:64-67):
+// phase / pi = 2 W + 2 sqrt(n^2 - (NA/M)^2 rho^2) z / lambda
+__device__ __forceinline__ double2 pupil_at(const PupilBase& b, double z, double lam) {
+  const double ph = b.w2 + b.kz2 * z / lam;
   double s, c;
   sincospi(ph, &s, &c);
-  return make_double2(ob * c, ob * s);
+  return make_double2(b.ob * c, b.ob * s);
 }
+
 
 __device__ __forceinline__ double2 dmul(double2 a, double2 b) {
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
@@ -134,7 +152,11 @@ __device__ void char_matrix(double f, double g, double na, double n, double2* M6
 // Grid (window blocks, source points): workgroup (x, y) takes window entries
 // x*256 .. x*256+255 of source points y, y + gridDim.y, ..., so the point's
 // values are uniform over the workgroup and no 64-bit division is needed.
-__global__ __launch_bounds__(256) void k_pupil(PupilParams P, const double* __restrict__ sx,
+// The focus-independent factors (obliquity, wavefront, polarisation) are
+// formed once per entry and the nfoci kernel sets written from them:
+// G[f][c][s][i][j] at the focus P.foci[f], f < nfoci <= kMaxFoci.
+__global__ __launch_bounds__(256) void k_pupil(PupilParams P, int nfoci,
+                                               const double* __restrict__ sx,
                                                const double* __restrict__ sy,
                                                const double* __restrict__ scale,
                                                const double2* __restrict__ px,
@@ -144,33 +166,39 @@ __global__ __launch_bounds__(256) void k_pupil(PupilParams P, const double* __re
   const int rem = blockIdx.x * blockDim.x + threadIdx.x;
   if (rem >= per) return;
   const int ncomp = P.vector ? 3 : 1;
+  const size_t set = (size_t)ncomp * nsrc * per;   // one focus's kernels
   const int i = rem / wc, j = rem - i * wc;
   for (int s = blockIdx.y; s < nsrc; s += gridDim.y) {
     const double f = (double)(k0r + i) / P.pr + sx[s];
     const double g = (double)(k0c + j) / P.pc + sy[s];
     const double rho2 = f * f + g * g;
-    double2 v[3] = {make_double2(0, 0), make_double2(0, 0), make_double2(0, 0)};
-    if (rho2 <= 1.0) {
-      const double2 p = pupil_value(P, f, g, rho2);
-      const double sc = scale[s];
-      if (P.vector) {
-        double2 M6[6];
-        char_matrix(f, g, P.na, P.n, M6);
-        const double2 ex = px[s], ey = py[s];
-        for (int c = 0; c < 3; ++c) {
-          // pol_c = px M_xc + py M_yc with (M_xc, M_yc) = (M6[0],M6[1]), (M6[2],M6[3]), (M6[4],M6[5])
-          const double2 t0 = dmul(ex, M6[2 * c]);
-          const double2 t1 = dmul(ey, M6[2 * c + 1]);
-          const double2 pol = make_double2(t0.x + t1.x, t0.y + t1.y);
-          const double2 q = dmul(pol, p);
-          v[c] = make_double2(q.x * sc, q.y * sc);
-        }
-      } else {
-        v[0] = make_double2(p.x * sc, p.y * sc);
+    float2* out = G + (size_t)s * per + rem;
+    if (!(rho2 <= 1.0)) {
+      for (int z = 0; z < nfoci; ++z)
+        for (int c = 0; c < ncomp; ++c) out[z * set + (size_t)c * nsrc * per] = make_float2(0.f, 0.f);
+      continue;
+    }
+    const PupilBase b = pupil_base(P, f, g, rho2);
+    const double sc = scale[s];
+    double2 pol[3] = {make_double2(1.0, 0.0), make_double2(0, 0), make_double2(0, 0)};
+    if (P.vector) {
+      double2 M6[6];
+      char_matrix(f, g, P.na, P.n, M6);
+      const double2 ex = px[s], ey = py[s];
+      for (int c = 0; c < 3; ++c) {
+        // pol_c = px M_xc + py M_yc with (M_xc, M_yc) = (M6[0],M6[1]), (M6[2],M6[3]), (M6[4],M6[5])
+        const double2 t0 = dmul(ex, M6[2 * c]);
+        const double2 t1 = dmul(ey, M6[2 * c + 1]);
+        pol[c] = make_double2(t0.x + t1.x, t0.y + t1.y);
       }
     }
-    for (int c = 0; c < ncomp; ++c)
-      G[((size_t)c * nsrc + s) * per + rem] = make_float2((float)v[c].x, (float)v[c].y);
+    for (int z = 0; z < nfoci; ++z) {
+      const double2 p = pupil_at(b, P.foci[z], P.lam);
+      for (int c = 0; c < ncomp; ++c) {
+        const double2 q = P.vector ? dmul(pol[c], p) : p;
+        out[z * set + (size_t)c * nsrc * per] = make_float2((float)(q.x * sc), (float)(q.y * sc));
+      }
+    }
   }
 }
 
@@ -202,14 +230,17 @@ static int blocks_for(size_t total) {
 
 using namespace oi;
 
-extern "C" int oi_pupil_kernels(const oi_optics* opt, const double* sx, const double* sy,
-                                const double* scale, const double* px, const double* py, int nsrc,
-                                int k0r, int wr, int k0c, int wc, void* G, void* stream) {
+static int launch_pupil(const char* who, const oi_optics* opt, const double* foci, int nfoci,
+                        const double* sx, const double* sy, const double* scale, const double* px,
+                        const double* py, int nsrc, int k0r, int wr, int k0c, int wc, void* G,
+                        void* stream) {
+  const std::string w(who);
   if (!opt || !sx || !sy || !scale || !G || nsrc < 1 || wr < 1 || wc < 1)
-    return fail(OI_ERR_INVALID, "pupil_kernels: bad arguments");
-  if (opt->vector && (!px || !py)) return fail(OI_ERR_INVALID, "pupil_kernels: vector mode needs px, py");
+    return fail(OI_ERR_INVALID, w + ": bad arguments");
+  if (opt->vector && (!px || !py)) return fail(OI_ERR_INVALID, w + ": vector mode needs px, py");
   if (!(opt->pitch_r > 0) || !(opt->pitch_c > 0) || !(opt->wavelength > 0))
-    return fail(OI_ERR_INVALID, "pupil_kernels: pitches and wavelength must be positive");
+    return fail(OI_ERR_INVALID, w + ": pitches and wavelength must be positive");
+  if ((size_t)wr * wc > (size_t)INT_MAX - 256) return fail(OI_ERR_INVALID, w + ": window too large");
   PupilParams P;
   P.z = fold_zernike(opt->zernike);
   P.lam = opt->wavelength;
@@ -220,15 +251,33 @@ extern "C" int oi_pupil_kernels(const oi_optics* opt, const double* sx, const do
   P.pr = opt->pitch_r;
   P.pc = opt->pitch_c;
   P.vector = opt->vector ? 1 : 0;
-  if ((size_t)wr * wc > (size_t)INT_MAX - 256)
-    return fail(OI_ERR_INVALID, "pupil_kernels: window too large");
   const dim3 grid((unsigned)((wr * wc + 255) / 256), (unsigned)(nsrc < 65535 ? nsrc : 65535));
-  hipLaunchKernelGGL(k_pupil, grid, dim3(256), 0,
-                     (hipStream_t)stream, P, sx, sy, scale, reinterpret_cast<const double2*>(px),
-                     reinterpret_cast<const double2*>(py), nsrc, k0r, wr, k0c, wc,
-                     static_cast<float2*>(G));
-  OI_LAUNCH_CHECK("pupil_kernels");
+  const size_t set = (size_t)(P.vector ? 3 : 1) * nsrc * wr * wc;   // one focus's kernels
+  for (int z0 = 0; z0 < nfoci; z0 += kMaxFoci) {
+    const int nz = nfoci - z0 < kMaxFoci ? nfoci - z0 : kMaxFoci;
+    for (int z = 0; z < kMaxFoci; ++z) P.foci[z] = z < nz ? (foci ? foci[z0 + z] : opt->focus) : 0.0;
+    hipLaunchKernelGGL(k_pupil, grid, dim3(256), 0, (hipStream_t)stream, P, nz, sx, sy, scale,
+                       reinterpret_cast<const double2*>(px), reinterpret_cast<const double2*>(py),
+                       nsrc, k0r, wr, k0c, wc, static_cast<float2*>(G) + (size_t)z0 * set);
+    OI_LAUNCH_CHECK("pupil_kernels");
+  }
   return OI_OK;
+}
+
+extern "C" int oi_pupil_kernels(const oi_optics* opt, const double* sx, const double* sy,
+                                const double* scale, const double* px, const double* py, int nsrc,
+                                int k0r, int wr, int k0c, int wc, void* G, void* stream) {
+  return launch_pupil("pupil_kernels", opt, nullptr, 1, sx, sy, scale, px, py, nsrc, k0r, wr, k0c,
+                      wc, G, stream);
+}
+
+extern "C" int oi_pupil_kernels_foci(const oi_optics* opt, const double* foci, int nfoci,
+                                     const double* sx, const double* sy, const double* scale,
+                                     const double* px, const double* py, int nsrc, int k0r, int wr,
+                                     int k0c, int wc, void* G, void* stream) {
+  if (!foci || nfoci < 1) return fail(OI_ERR_INVALID, "pupil_kernels_foci: foci must hold nfoci >= 1 values");
+  return launch_pupil("pupil_kernels_foci", opt, foci, nfoci, sx, sy, scale, px, py, nsrc, k0r, wr,
+                      k0c, wc, G, stream);
 }
 
 extern "C" int oi_zernike(const float* rho, const float* theta, int n, const double* coeffs,

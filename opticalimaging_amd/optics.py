@@ -158,6 +158,26 @@ def pupil_kernels(src: SourceSet, opt: nat.Optics, scale: torch.Tensor, k0r, wr,
     return G
 
 
+def pupil_kernels_foci(src: SourceSet, opt: nat.Optics, foci, scale: torch.Tensor, k0r, wr, k0c,
+                       wc):
+    """G[f][e][s][i][j] (c64): pupil_kernels at each focus of `foci` (opt.focus
+    is not read), the focus-independent factors evaluated once
+    (oi_pupil_kernels_foci)."""
+    dev = src.V.device
+    ncomp = 3 if opt.vector else 1
+    zs = [float(z) for z in foci]
+    G = torch.empty((len(zs), ncomp, src.n, wr, wc), dtype=torch.complex64, device=dev)
+    scale = scale.to(device=dev, dtype=torch.float64).contiguous()
+    px = torch.view_as_real(src.PX).contiguous() if opt.vector else None
+    py = torch.view_as_real(src.PY).contiguous() if opt.vector else None
+    zarr = (ctypes.c_double * len(zs))(*zs)
+    nat.check(nat.load().oi_pupil_kernels_foci(ctypes.byref(opt), zarr, len(zs), nat.ptr(src.X),
+                                               nat.ptr(src.Y), nat.ptr(scale), nat.ptr(px),
+                                               nat.ptr(py), src.n, k0r, wr, k0c, wc, nat.ptr(G),
+                                               nat.stream()))
+    return G
+
+
 def tcc_halfwidths(xypitch, NA, wavelength, numerics=None):
     """(Nf, Ng) of the TCC grid: ceil(2 p) decided in float32 as
     CalculateTCCMatrix.py:17-20 does, unless numerics.Hopkins_GridHalfWidth

@@ -85,3 +85,28 @@ def test_serving_all_sizes_8gb(gpu):
         err = rel_err(mixed[i], ref)
         print(f"served {n}^2 request {i} vs float64 SOCS (K = {ks.K}): {err:.2e}", flush=True)
         assert err < 1e-5
+
+
+@pytest.mark.parametrize("mode,nfoci", [("vector", 5), ("scalar", 37)])
+def test_multi_focus_pupils_equal_per_focus_pupils(gpu, mode, nfoci):
+    """oi_pupil_kernels_foci writes, for every focus, the kernels that
+    oi_pupil_kernels writes for that focus alone, bit for bit; 37 foci cross
+    the 32 foci that one launch carries."""
+    from opticalimaging_amd import optics, workloads
+    from opticalimaging_amd.Scatter import _pitches
+    mask, src, proj, num, period = workloads.vector_abbe_512(n=128)
+    sc = scatter(workloads.field_from_mask(mask).cuda(), period, period)
+    sset = optics.source_set(src)
+    px, py = _pitches(sc, proj, src)
+    k0r, wr, k0c, wc, _, _ = optics.abbe_window(sset, py, px, 128, 128)
+    vector = mode == "vector"
+    foci = [-75.0 + 150.0 * i / (nfoci - 1) for i in range(nfoci)]
+    scale = torch.sqrt(float(proj.IndexImage) * sset.V / sset.weight)
+    opt = optics.optics_record(src, proj, 0.0, py, px, vector)
+    G = optics.pupil_kernels_foci(sset, opt, foci, scale, k0r, wr, k0c, wc)
+    assert G.shape == (nfoci, 3 if vector else 1, sset.n, wr, wc)
+    for i, z in enumerate(foci):
+        one = optics.pupil_kernels(sset, optics.optics_record(src, proj, z, py, px, vector), scale,
+                                   k0r, wr, k0c, wc)
+        assert torch.equal(torch.view_as_real(G[i]), torch.view_as_real(one)), (mode, i, z)
+    assert not torch.equal(G[0], G[-1])                 # the defocus phase is live

@@ -234,8 +234,11 @@ static int plan_geometry(const oi_geometry& in, oi_geometry& g) {
     // 512 workgroups for the generic kernels
     const int phases = lr / g.rows;
     const int target = g.path == 1 ? 256 : 512;
+    // one image fills the CUs with up to 256 one-pass groups (its kernels
+    // over the groups); the generic kernels stop at 64
+    const int cap = g.path == 1 ? 256 : 64;
     int gr = (target + phases * g.max_batch - 1) / (phases * g.max_batch);
-    g.max_groups = gr < 1 ? 1 : (gr > 64 ? 64 : gr);
+    g.max_groups = gr < 1 ? 1 : (gr > cap ? cap : gr);
   }
   return OI_OK;
 }

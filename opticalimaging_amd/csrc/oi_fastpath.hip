@@ -752,7 +752,7 @@ __global__ __launch_bounds__(kBlock, 4) void k_fp_brows(const float2* __restrict
 // Ihat[img][q][i] = scale * sum_t (sum_g R[img][g][q][t]) e^{+2 pi i qr t / L},
 // qr = i - qr_max, i in [0, 2 qr_max + 1).
 // One order q per workgroup of kBandWide threads.  The sum over the kernel
-// groups (up to 64 for a batch of one image) is spread over the threads:
+// groups (up to 256 for a batch of one image) is spread over the threads:
 // S = kBandWide / L slices of the groups, each slice summed in order, then the
 // slices in order, so that it never runs as `groups` dependent loads in a
 // handful of threads; then one group of threads transforms the column.

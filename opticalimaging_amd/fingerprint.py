@@ -17,12 +17,25 @@ import torch
 __all__ = ["canon", "params", "digest"]
 
 
+_DTYPE_NAMES = {}
+
+
+def _dtype_name(dt) -> str:
+    name = _DTYPE_NAMES.get(dt)
+    if name is None:
+        name = _DTYPE_NAMES[dt] = str(dt)
+    return name
+
+
 def canon(v, depth=0):
     """A JSON-able form of a parameter value; arrays enter by shape, dtype
     and a digest of their bytes."""
     if depth > 4:
         raise ValueError("parameter objects nest too deeply to fingerprint")
-    if v is None or isinstance(v, (bool, int, str)):
+    t = type(v)
+    if t is float:
+        return repr(v)
+    if v is None or t is int or t is str or t is bool or isinstance(v, (bool, int, str)):
         return v
     if isinstance(v, float):
         return repr(v)
@@ -32,7 +45,7 @@ def canon(v, depth=0):
         v = v.detach().cpu().contiguous().numpy()
     if isinstance(v, np.ndarray):
         a = np.ascontiguousarray(v)
-        return {"shape": list(a.shape), "dtype": str(a.dtype),
+        return {"shape": list(a.shape), "dtype": _dtype_name(a.dtype),
                 "sha256": hashlib.sha256(a.tobytes()).hexdigest()}
     if isinstance(v, np.generic):
         return canon(v.item(), depth)

@@ -62,7 +62,7 @@ def abbe_images(field, source, scatter, projector, numerics, encoding=None) -> t
     nat.check(lib.oi_spectrum_window(plan.handle, nat.ptr(tens), kind, cin, cptr, 1, nat.ptr(spec),
                                      nat.stream()))
     del keep
-    scale = torch.sqrt(float(projector.IndexImage) * src.V / src.weight)
+    scale = src.kernel_scale(projector.IndexImage)
     foci = optics.focus_values(projector)
     out = torch.empty((len(foci), ny, nx), dtype=torch.float32, device=dev)
     opt = optics.optics_record(source, projector, 0.0, py, px, vector)

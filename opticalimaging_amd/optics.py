@@ -6,7 +6,7 @@ import collections
 import ctypes
 import math
 import threading
-from dataclasses import dataclass
+from dataclasses import dataclass, field
 
 import numpy as np
 import torch
@@ -46,10 +46,21 @@ class SourceSet:
     PY: torch.Tensor
     extent: tuple = None       # (xmin, xmax, ymin, ymax)
     total: float = None        # sum of V
+    _scales: dict = field(default_factory=dict, repr=False, compare=False)
 
     @property
     def n(self) -> int:
         return int(self.V.numel())
+
+    def kernel_scale(self, index_image: float) -> torch.Tensor:
+        """sqrt(n V_s / W) per point, the weight of each source point's
+        coherent kernel (CalculateAerialImage.py:89-92 folded into the
+        kernels); kept per image-side index.  Read only."""
+        key = float(index_image)
+        sc = self._scales.get(key)
+        if sc is None:
+            sc = self._scales[key] = torch.sqrt(key * self.V / self.weight)
+        return sc
 
     @property
     def weight(self) -> float:

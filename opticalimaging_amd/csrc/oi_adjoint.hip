@@ -339,7 +339,8 @@ __global__ __launch_bounds__(kRegBlock) void k_adj_cwin_reg(const float2* __rest
 // dividing the larger), as the fast imaging path's coarse grids.
 #define OI_ADJ_PLANS(X)                                                                        \
   X(16, 4, 4) X(25, 5, 5) X(32, 4, 8) X(36, 6, 6) X(48, 4, 12) X(64, 8, 8) X(75, 5, 15)       \
-  X(81, 9, 9) X(100, 10, 10) X(128, 8, 16) X(144, 12, 12) X(256, 16, 16) X(512, 16, 32)
+  X(81, 9, 9) X(100, 10, 10) X(128, 8, 16) X(144, 12, 12) X(256, 16, 16) X(450, 15, 30)     \
+  X(512, 16, 32)
 
 static bool adj_reg_size(int L) {
 #define X(NN, A, B) if (L == NN) return true;
@@ -394,7 +395,7 @@ static int adj_rfield_reg(int L, dim3 grid_y_z, float2* Ec, int nk, int k0c, int
 }
 
 // Sizes of the forward's two-pass column buffer.
-#define OI_ADJ_BCOLS_PLANS(X) X(144, 12, 12) X(256, 16, 16) X(512, 16, 32)
+#define OI_ADJ_BCOLS_PLANS(X) X(144, 12, 12) X(256, 16, 16) X(450, 15, 30) X(512, 16, 32)
 
 static bool adj_bcols_size(int L) {
 #define X(NN, A, B) if (L == NN) return true;

@@ -1222,7 +1222,7 @@ static int run_band(const FpArgs& a, int groups, hipStream_t st) {
 }
 
 // Large coarse grids of the two-pass coherent sum.
-#define OI_TWO_PASS_PLANS(X) X(144, 12, 12) X(256, 16, 16) X(512, 16, 32)
+#define OI_TWO_PASS_PLANS(X) X(144, 12, 12) X(256, 16, 16) X(450, 15, 30) X(512, 16, 32)
 
 template <class PL>
 static size_t two_pass_rows_lds(int wc) {

@@ -100,19 +100,22 @@ def test_fast_abbe_1024_config3_optics(gpu):
     assert err < 1e-5
 
 
-@pytest.mark.parametrize("n", [1024, 2048])
-def test_two_pass_large_coarse_grid(gpu, n):
+@pytest.mark.parametrize("n,pixel,coarse", [(1024, 4.0, 256), (2048, 4.0, 450), (2048, 4.4, 512)])
+def test_two_pass_large_coarse_grid(gpu, n, pixel, coarse):
     """Kernels of the NA 1.35 quasar optics on 4 nm pixels: windows of 109 / 217
-    orders need coarse grids of 256 / 512 (two-pass coherent sum); 2048^2
-    images take the three-stage register transforms (16 x 8 x 16)."""
+    orders need coarse grids of at least 217 / 433 points, and take 256 / 450
+    (= 15 x 30), the smallest two-pass grids that hold them; at 4.4 nm the
+    2048^2 window needs more than 450 and takes 512.  2048^2 images take the
+    three-stage register transforms (16 x 8 x 16)."""
     from opticalimaging_amd import engine, workloads
     from opticalimaging_amd.DecomposeTCC_SOCS import socs_kernel_set
     src, proj, num = workloads.ilt_optics()
-    period = 4.0 * n
+    period = pixel * n
     ks = socs_kernel_set(src, [period, period], proj, 0.0, num, [n, n])
     plan = engine.plan_for(ks, 1, 2)
     info = plan.info()
-    assert info["path"] == 2 and info["lr"] == (256 if n == 1024 else 512), info
+    assert info["path"] == 2 and info["lr"] == coarse, info
+    assert 2 * ks.wr - 1 <= coarse
     masks = np.stack([workloads.serving_mask(s, n) for s in (5, 6)])
     img = engine.socs_image(torch.as_tensor(masks).cuda(), ks, plan=plan).cpu().numpy()
     dense = ks.dense().cpu().numpy().astype(np.complex128)

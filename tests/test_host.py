@@ -163,9 +163,10 @@ def test_workloads_deterministic():
 
 
 def test_band_step_and_bit_decode_reject_arguments_before_any_device_call():
-    """The argument checks of the band-form ILT entry points and of the bit
-    decode run on the host, ahead of any HIP call: a null pointer, a gradient
-    buffer that is the logits buffer itself, a misaligned bit buffer."""
+    """The argument checks of the band-form ILT entry points, the bit decode
+    and the multi-focus pupils run on the host, ahead of any HIP call: a null
+    pointer, a gradient buffer that is the logits buffer itself, a misaligned
+    bit buffer, an empty focus list, zero pitches."""
     import ctypes
     from opticalimaging_amd import _native
     lib = _native.load()
@@ -188,6 +189,16 @@ def test_band_step_and_bit_decode_reject_arguments_before_any_device_call():
     rc = lib.oi_mask_decode_bits(ctypes.c_void_p(ctypes.addressof(buf) + 2), 32, levels, a, null)
     assert rc == _native.OI_ERR_INVALID
     with pytest.raises(ValueError, match="4-byte"):
+        _native.check(rc)
+    opt = _native.Optics()
+    zs = (ctypes.c_double * 2)(0.0, 50.0)
+    assert lib.oi_pupil_kernels_foci(ctypes.byref(opt), None, 2, a, a, a, null, null, 4, 0, 3, 0, 3, a,
+                                     null) == _native.OI_ERR_INVALID
+    assert lib.oi_pupil_kernels_foci(ctypes.byref(opt), zs, 0, a, a, a, null, null, 4, 0, 3, 0, 3, a,
+                                     null) == _native.OI_ERR_INVALID
+    rc = lib.oi_pupil_kernels_foci(ctypes.byref(opt), zs, 2, a, a, a, null, null, 4, 0, 3, 0, 3, a, null)
+    assert rc == _native.OI_ERR_INVALID                          # zero pitches and wavelength
+    with pytest.raises(ValueError, match="pitches and wavelength"):
         _native.check(rc)
     al = ctypes.c_void_p((ctypes.addressof(buf) + 15) & ~15)
     assert lib.oi_mask_decode_bits(al, 0, levels, al, null) == _native.OI_OK   # nothing to decode

@@ -773,11 +773,26 @@ __global__ __launch_bounds__(kBandWide) void k_fp_band(const float2* __restrict_
   const int img = blockIdx.y, q = blockIdx.x;
   const int nq = qc + 1;
   const float2* src = R + ((size_t)img * groups * nq + q) * L;
+  const size_t gstride = (size_t)nq * L;
   for (int i = threadIdx.x; i < S * L; i += kBandWide) {
     const int gs = i / L, t = i - gs * L;
     float2 a = make_float2(0.f, 0.f);
-    for (int k = gs; k < groups; k += S) {
-      const float2 x = src[(size_t)k * nq * L + t];
+    int k = gs;
+    // eight loads in flight, then their sums in the same order as one at a
+    // time: a slice of 64 groups (256 groups of a one-image plan) is not a
+    // chain of 64 dependent load latencies
+    for (; k + 7 * S < groups; k += 8 * S) {
+      float2 x[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) x[u] = src[(size_t)(k + u * S) * gstride + t];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        a.x += x[u].x;
+        a.y += x[u].y;
+      }
+    }
+    for (; k < groups; k += S) {
+      const float2 x = src[(size_t)k * gstride + t];
       a.x += x.x;
       a.y += x.y;
     }

@@ -82,6 +82,8 @@ def parse():
                     help="only run the ILT leg and print its JSON (profiling runs)")
     ap.add_argument("--abbe-only", action="store_true",
                     help="only run the two Abbe configurations and print their JSON (profiling runs)")
+    ap.add_argument("--serving-only", action="store_true",
+                    help="only run the serving leg and print its JSON (profiling runs)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 device copies of MIB MiB (calibrates HBM byte counters)")
     return ap.parse_args()
@@ -466,6 +468,9 @@ def main():
         return
     if args.abbe_only:
         print(json.dumps({"abbe_configs": abbe_configs_leg()}), flush=True)
+        return
+    if args.serving_only:
+        print(json.dumps({"serving": serving_leg(args.serving_requests, args.serving_rate)}), flush=True)
         return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:

@@ -679,10 +679,18 @@ __global__ __launch_bounds__(kBlock) void k_fp_bcols(const float2* __restrict__ 
 // Blocks (row group, img, grp) sum the (kernel, channel) pairs kc = grp,
 // grp + groups, ... into their own partial R[img][grp] (groups = 1 for the
 // generic layout).
-// Four waves per SIMD (at most 128 registers): at coarse 256 four workgroups
-// then share a CU, as their 34 KB of LDS allow (ILT step 0.355 -> 0.348 ms).
+// Four waves per SIMD (at most 128 registers) up to coarse 256: four
+// workgroups then share a CU, as their 34 KB of LDS allow (ILT step 0.355 ->
+// 0.348 ms).  From 450 on the LDS (about 70 KB) holds two workgroups per CU
+// whatever the registers, and the 30- and 32-value transforms would spill
+// under 128: two waves per SIMD (up to 256 registers).
 template <class PL>
-__global__ __launch_bounds__(kBlock, 4) void k_fp_brows(const float2* __restrict__ B, int nkc, int k0c,
+constexpr int brows_min_blocks() {
+  return PL::N > 256 ? 2 : 4;
+}
+
+template <class PL>
+__global__ __launch_bounds__(kBlock, brows_min_blocks<PL>()) void k_fp_brows(const float2* __restrict__ B, int nkc, int k0c,
                                                      int wc, int qc, int generic_layout,
                                                      const float2* __restrict__ twt,
                                                      float2* __restrict__ R, int batch,
